@@ -1,0 +1,213 @@
+/*
+ * prolix_gb.h -- C ABI of the MI355X (gfx950) GB-OBC2 implicit-solvent MD engine.
+ *
+ * Plain C types only: host pointers for topology/parameters, device pointers
+ * for per-step arrays, and an opaque `void*` that is a hipStream_t (NULL =
+ * the device's legacy default stream). Captured step graphs replay on the
+ * engine's own stream, ordered after and before the caller's stream by events.
+ * Every function returns PRX_OK (0) or a PRX_ERR_* code; prx_last_error()
+ * returns a thread-local message for the last failure on the calling thread.
+ * No C++ exception crosses this boundary.
+ *
+ * Each entry point names the prolix interface it replaces (paths under
+ * prolix's src/prolix/).
+ *
+ * Units: A, kcal/mol, e, amu; time in fs / ps at the boundary (AKMA inside).
+ * Threading: a handle is used from one host thread at a time.
+ */
+#ifndef PROLIX_GB_H
+#define PROLIX_GB_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PRX_ABI_VERSION 1
+
+enum prx_status {
+  PRX_OK = 0,
+  PRX_ERR_INVALID = 1,  /* bad argument, shape or index */
+  PRX_ERR_HIP = 2,      /* a HIP runtime call failed */
+  PRX_ERR_CAPACITY = 3, /* pool capacity / HBM budget exhausted */
+  PRX_ERR_STATE = 4     /* call out of order (e.g. steps before init) */
+};
+
+/* Per-system energy outputs, the terms summed by total_energy
+ * (physics/system.py:229-248). energies[] rows are PRX_N_ENERGY doubles. */
+enum prx_energy_term {
+  PRX_E_BOND = 0,
+  PRX_E_ANGLE = 1,
+  PRX_E_UREY_BRADLEY = 2,
+  PRX_E_DIHEDRAL = 3,
+  PRX_E_IMPROPER = 4,
+  PRX_E_CMAP = 5,
+  PRX_E_LJ = 6,
+  PRX_E_EXCEPTION = 7,
+  PRX_E_GB = 8,
+  PRX_E_COULOMB = 9,
+  PRX_E_ACE = 10,
+  PRX_E_TOTAL = 11,
+  PRX_N_ENERGY = 12
+};
+
+/* One system's topology and parameters (host arrays, read during the call).
+ * Mirrors PhysicsSystem (typing.py:248-310) plus the make_energy_fn options
+ * (physics/system.py:29-37, 110-135). NULL pointers with a zero count are
+ * fine for every optional term. */
+typedef struct prx_system_desc {
+  int32_t n_atoms;
+  const double* charges;      /* (n) e */
+  const double* sigmas;       /* (n) A, clamped to >= 1e-6 (system.py:107) */
+  const double* epsilons;     /* (n) kcal/mol */
+  const double* radii;        /* (n) A, OBC intrinsic radii */
+  const double* scaled_radii; /* (n) A absolute descreening radii; NULL -> radii */
+  const double* masses;       /* (n) amu; NULL -> 1 */
+
+  int32_t n_bonds;            /* harmonic 1/2 k (r - r0)^2, params [r0, k] */
+  const int32_t* bond_idx;    /* (n_bonds, 2) */
+  const double* bond_params;  /* (n_bonds, 2) */
+  int32_t n_ub;               /* Urey-Bradley, same form as bonds */
+  const int32_t* ub_idx;
+  const double* ub_params;
+  int32_t n_angles;           /* 1/2 k (theta - theta0)^2, params [theta0, k] */
+  const int32_t* angle_idx;   /* (n_angles, 3) */
+  const double* angle_params; /* (n_angles, 2) */
+  int32_t n_dihedrals;        /* sum_t k (1 + cos(n phi - phase)), params [n, phase, k] */
+  int32_t dihedral_terms;     /* T */
+  const int32_t* dihedral_idx;   /* (n_dihedrals, 4) */
+  const double* dihedral_params; /* (n_dihedrals, T, 3) */
+  int32_t n_impropers;
+  int32_t improper_terms;        /* T */
+  int32_t improper_param_width;  /* 3: periodic [n, phase, k]; 2: harmonic [k, phi0] */
+  const int32_t* improper_idx;   /* (n_impropers, 4) */
+  const double* improper_params; /* (n_impropers, T, width) */
+  int32_t n_cmap;
+  const int32_t* cmap_torsions;  /* (n_cmap, 5): phi = atoms 0..3, psi = atoms 1..4 */
+  const int32_t* cmap_map;       /* (n_cmap) map index */
+  int32_t n_cmap_maps;
+  int32_t cmap_grid;             /* G (24) */
+  const double* cmap_grids;      /* (n_cmap_maps, G, G) raw energies stored [psi, phi] */
+
+  int32_t excl_cols;             /* C of the (n, C) exclusion rows (0 = no exclusions) */
+  const int32_t* excl_idx;       /* (n, C) partner index, -1 = empty */
+  const double* excl_scale_vdw;  /* (n, C) */
+  const double* excl_scale_elec; /* (n, C) */
+
+  int32_t n_exceptions;          /* explicit 1-4 pairs (bonded.py:188-228) */
+  const int32_t* exc_pairs;      /* (n_exceptions, 2) */
+  const double* exc_sigma;
+  const double* exc_epsilon;
+  const double* exc_chargeprod;
+
+  double cutoff;              /* A; 0 = no cutoff. LJ/Coulomb only, GB is never cut */
+  double coulomb_constant;    /* 332.0637 (system.py:135) */
+  double gb_coulomb_constant; /* constant of the GB prefactor */
+  double solute_dielectric;   /* 1.0 */
+  double solvent_dielectric;  /* 78.5 */
+  double dielectric_offset;   /* 0.09 A */
+  int32_t implicit_solvent;   /* 1: GB-OBC2 + ACE on */
+} prx_system_desc;
+
+/* Langevin BAOAB parameters for one system (jax_md nvt_langevin as used at
+ * physics/simulate.py:245-255; AKMA conversion as simulate.py:770-772). */
+typedef struct prx_langevin_params {
+  double dt_fs;       /* time step, fs */
+  double temperature; /* K */
+  double gamma_ps;    /* friction, 1/ps (0 = velocity Verlet) */
+  uint64_t seed;      /* Philox4x32-10 key of this system's noise stream */
+} prx_langevin_params;
+
+typedef struct prx_engine prx_engine;
+
+/* ---- library ---------------------------------------------------------- */
+
+/* Message for the last failed call on this thread ("" if none). */
+const char* prx_last_error(void);
+int32_t prx_abi_version(void);
+
+/* ---- systems: make_energy_fn's set-up (physics/system.py:29-171) --------
+ * Upload n_systems systems packed without padding into one engine. The
+ * engine owns every device buffer until prx_engine_destroy. */
+int prx_system_create(const prx_system_desc* descs, int32_t n_systems, prx_engine** out);
+int prx_engine_destroy(prx_engine* eng);
+
+/* Packed layout: offsets[s] = first atom of system s, offsets[n] = total.
+ * The atom order inside a system is the caller's. */
+int prx_engine_layout(const prx_engine* eng, int32_t* atom_offsets, int32_t capacity);
+
+/* ---- energy / forces: total_energy(r) and -jax.grad of it -------------
+ * Replaces make_energy_fn's total_energy (physics/system.py:229-248) and
+ * its gradient (custom_vjp bwd of optimization.py:75-126, 250-297 plus the
+ * Born-radius chain rule, generalized_born.py:651-817).
+ * positions/forces: device (total_atoms, 3) float32 in packed order.
+ * energies: device (n_systems, PRX_N_ENERGY) float64, or NULL.
+ * born_radii: device (total_atoms) float32, or NULL. Stream-ordered. */
+int prx_energy_forces(prx_engine* eng, const float* positions, float* forces, double* energies,
+                      float* born_radii, void* stream);
+
+/* ---- integrator: nvt_langevin init_fn / apply_fn ------------------------
+ * prx_integrator_init = init_fn(key, R, mass) (settle.py:1676-1697):
+ * positions (device, packed) become the state; momenta (device, packed) or
+ * NULL to draw Maxwell-Boltzmann momenta from each system's Philox stream.
+ * params: host array with one entry per system. Computes the initial forces. */
+int prx_integrator_init(prx_engine* eng, const float* positions, const float* momenta,
+                        const prx_langevin_params* params, void* stream);
+/* n_steps BAOAB steps of every system (settle.py:1699-1744, jax_md
+ * nvt_langevin apply_fn), replayed from a captured hipGraph. */
+int prx_integrator_steps(prx_engine* eng, int32_t n_steps, void* stream);
+/* Copy state out (device pointers, packed; any may be NULL). */
+int prx_integrator_state(prx_engine* eng, float* positions, float* momenta, float* forces,
+                         void* stream);
+/* Per-system kinetic energy sum p^2 / 2m (kcal/mol), float64 reduction. */
+int prx_integrator_kinetic(prx_engine* eng, double* kinetic, void* stream);
+/* Steps taken so far by system s (host value, synchronises the stream). */
+int prx_integrator_step_count(prx_engine* eng, int32_t system, int64_t* steps);
+
+/* ---- serving pool -------------------------------------------------------
+ * A pool is an engine whose systems come and go. Capacities bound the
+ * device arenas (atoms, bonded entries, CMAP coefficients, exclusion tiles);
+ * prx_pool_admit fails with PRX_ERR_CAPACITY when a system does not fit.
+ * No counterpart in prolix, whose batched path pads to buckets
+ * (padding.py:573-654, batched_simulate.py:1125-1422). */
+typedef struct prx_pool_limits {
+  int64_t max_atoms;
+  int64_t max_bonded;     /* bonds + UB + angles + torsion terms + CMAP + exceptions */
+  int64_t max_cmap_coeff; /* floats of CMAP patch coefficients (16 per patch) */
+  int64_t max_excl_tiles; /* 64x64 tiles that hold exclusions */
+  int32_t max_systems;
+} prx_pool_limits;
+
+int prx_pool_create(const prx_pool_limits* limits, prx_engine** out);
+/* Admit one system with its start positions (host, (n,3) float32) and
+ * integrator parameters; momenta are drawn from its Philox stream. The new
+ * system joins the next prx_integrator_steps call. slot receives its id. */
+int prx_pool_admit(prx_engine* pool, const prx_system_desc* desc, const float* positions_host,
+                   const prx_langevin_params* params, int32_t* slot, void* stream);
+/* Remove a system; its arena ranges are reused by later admissions. */
+int prx_pool_release(prx_engine* pool, int32_t slot, void* stream);
+/* Copy one system's positions / momenta / forces to host ((n,3) float32 each,
+ * any may be NULL) and its step count; synchronises the stream. */
+int prx_pool_read(prx_engine* pool, int32_t slot, float* positions, float* momenta, float* forces,
+                  int64_t* steps, void* stream);
+/* Number of live systems and their slot ids (capacity entries max). */
+int prx_pool_slots(const prx_engine* pool, int32_t* slots, int32_t capacity, int32_t* n_live);
+/* Energies of one live slot at its current positions (host out, PRX_N_ENERGY doubles). */
+int prx_pool_energy(prx_engine* pool, int32_t slot, double* energies_host, void* stream);
+
+/* ---- host-side planning (no device work; usable without a GPU) --------- */
+
+/* Bicubic CMAP patch coefficients of prolix's cmap.py:242-286 for one grid
+ * stored [psi, phi]: out is (G, G, 16) float64 indexed [phi, psi]. */
+int prx_cmap_coefficients(const double* grid, int32_t G, double* out);
+
+/* Number of 64x64 tile pairs (work items) and of exclusion tiles the pair
+ * kernels would use for one system. */
+int prx_plan_tiles(const prx_system_desc* desc, int64_t* n_items, int64_t* n_excl_tiles);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PROLIX_GB_H */

@@ -1,0 +1,235 @@
+// Bonded terms: one thread per arena entry, forces scattered into the same
+// 64-bit fixed-point accumulators as the pair passes.
+//
+//   bonds / Urey-Bradley  1/2 k (r - r0)^2             bonded.py:20-44
+//   angles                1/2 k (theta - theta0)^2     bonded.py:47-81
+//   torsion terms         k (1 + cos(n phi - phase))   bonded.py:84-141
+//   harmonic impropers    k wrap(phi - phi0)^2         bonded.py:144-185
+//   CMAP bicubic patches                               cmap.py:316-389
+//   1-4 exceptions        LJ 12-6 + Coulomb            bonded.py:188-228
+//
+// The grid is split into five consecutive block ranges, one per entry kind.
+#include "prx_common.h"
+
+namespace prx {
+
+constexpr int kBondedThreads = 256;
+
+__device__ __forceinline__ void scatter3(const DeviceView& v, int g, float fx, float fy, float fz) {
+  unsigned long long* a = &v.acc[(size_t)g * 4];
+  const long long qx = __double2ll_rn((double)fx * kForceScale);
+  const long long qy = __double2ll_rn((double)fy * kForceScale);
+  const long long qz = __double2ll_rn((double)fz * kForceScale);
+  if (qx) atomicAdd(a + 0, (unsigned long long)qx);
+  if (qy) atomicAdd(a + 1, (unsigned long long)qy);
+  if (qz) atomicAdd(a + 2, (unsigned long long)qz);
+}
+
+__device__ __forceinline__ float3 ld3(const DeviceView& v, int g) {
+  const float4 p = v.pos[g];
+  return make_float3(p.x, p.y, p.z);
+}
+__device__ __forceinline__ float3 sub3(float3 a, float3 b) { return make_float3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ float dot3(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float3 cross3(float3 a, float3 b) {
+  return make_float3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ float3 scl3(float s, float3 a) { return make_float3(s * a.x, s * a.y, s * a.z); }
+__device__ __forceinline__ float3 add3(float3 a, float3 b) { return make_float3(a.x + b.x, a.y + b.y, a.z + b.z); }
+
+// Dihedral angle as bonded.py:84-107 (b1 norm + 1e-8) and its gradient with
+// respect to the four atoms (Blondel & Karplus, J. Comput. Chem. 17, 1132).
+struct Dihedral {
+  float phi;
+  float3 g0, g1, g2, g3;
+};
+
+__device__ __forceinline__ Dihedral dihedral(float3 p0, float3 p1, float3 p2, float3 p3) {
+  const float3 b0 = sub3(p0, p1), b1 = sub3(p2, p1), b2 = sub3(p3, p2);
+  const float b1n = sqrtf(dot3(b1, b1));
+  const float3 u = scl3(1.0f / (b1n + 1e-8f), b1);
+  const float3 vv = sub3(b0, scl3(dot3(b0, u), u));
+  const float3 ww = sub3(b2, scl3(dot3(b2, u), u));
+  Dihedral d;
+  d.phi = atan2f(dot3(cross3(u, vv), ww), dot3(vv, ww));
+  // F = p0 - p1, G = p1 - p2, H = p3 - p2
+  const float3 F = b0, G = scl3(-1.0f, b1), H = b2;
+  const float3 A = cross3(F, G), B = cross3(H, G);
+  const float A2 = dot3(A, A), B2 = dot3(B, B);
+  const float FG = dot3(F, G), HG = dot3(H, G);
+  const float ga = b1n / A2, gb = b1n / B2;
+  const float fa = FG / (A2 * b1n), hb = HG / (B2 * b1n);
+  d.g0 = scl3(-ga, A);
+  d.g3 = scl3(gb, B);
+  d.g1 = add3(scl3(ga + fa, A), scl3(-hb, B));
+  d.g2 = add3(scl3(-fa, A), scl3(hb - gb, B));
+  return d;
+}
+
+__device__ __forceinline__ void scatter_torsion(const DeviceView& v, int4 a, const Dihedral& d, float dEdphi) {
+  scatter3(v, a.x, -dEdphi * d.g0.x, -dEdphi * d.g0.y, -dEdphi * d.g0.z);
+  scatter3(v, a.y, -dEdphi * d.g1.x, -dEdphi * d.g1.y, -dEdphi * d.g1.z);
+  scatter3(v, a.z, -dEdphi * d.g2.x, -dEdphi * d.g2.y, -dEdphi * d.g2.z);
+  scatter3(v, a.w, -dEdphi * d.g3.x, -dEdphi * d.g3.y, -dEdphi * d.g3.z);
+}
+
+template <bool ENERGY>
+__device__ void do_bond(const DeviceView& v, int i) {
+  const BondEntry b = v.bonds[i];
+  if (b.idx.x < 0) return;
+  const float3 d = sub3(ld3(v, b.idx.x), ld3(v, b.idx.y));
+  const float r = sqrtf(dot3(d, d));
+  const float dr = r - b.p.x;
+  const float s = -b.p.y * dr / r;  // F_0 = s * d
+  scatter3(v, b.idx.x, s * d.x, s * d.y, s * d.z);
+  scatter3(v, b.idx.y, -s * d.x, -s * d.y, -s * d.z);
+  if (ENERGY) v.e_bond[i] = 0.5 * (double)b.p.y * (double)dr * (double)dr;
+}
+
+template <bool ENERGY>
+__device__ void do_angle(const DeviceView& v, int i) {
+  const AngleEntry a = v.angles[i];
+  if (a.idx.x < 0) return;
+  const float3 pj = ld3(v, a.idx.y);
+  const float3 v1 = sub3(ld3(v, a.idx.x), pj), v2 = sub3(ld3(v, a.idx.z), pj);
+  const float d1 = sqrtf(dot3(v1, v1)), d2 = sqrtf(dot3(v2, v2));
+  const float Q = d1 * d2 + 1e-8f;
+  const float P = dot3(v1, v2);
+  const float c = P / Q;
+  const float cc = fminf(fmaxf(c, -0.999999f), 0.999999f);
+  const float th = acosf(cc);
+  const float dth = th - a.p.x;
+  const bool inside = c > -0.999999f && c < 0.999999f;
+  const float dEdc = inside ? -a.p.y * dth / sqrtf(1.0f - cc * cc) : 0.0f;
+  const float k1 = P / (Q * Q);
+  const float3 g0 = sub3(scl3(1.0f / Q, v2), scl3(k1 * d2 / d1, v1));
+  const float3 g2 = sub3(scl3(1.0f / Q, v1), scl3(k1 * d1 / d2, v2));
+  scatter3(v, a.idx.x, -dEdc * g0.x, -dEdc * g0.y, -dEdc * g0.z);
+  scatter3(v, a.idx.z, -dEdc * g2.x, -dEdc * g2.y, -dEdc * g2.z);
+  scatter3(v, a.idx.y, dEdc * (g0.x + g2.x), dEdc * (g0.y + g2.y), dEdc * (g0.z + g2.z));
+  if (ENERGY) v.e_angle[i] = 0.5 * (double)a.p.y * (double)dth * (double)dth;
+}
+
+template <bool ENERGY>
+__device__ void do_torsion(const DeviceView& v, int i) {
+  const TorsionEntry t = v.tors[i];
+  if (t.idx.x < 0) return;
+  const Dihedral d = dihedral(ld3(v, t.idx.x), ld3(v, t.idx.y), ld3(v, t.idx.z), ld3(v, t.idx.w));
+  float e, dE;
+  if (t.p.w == 2.0f) {  // harmonic improper: p.x = k, p.y = phi0
+    const float twopi = 6.283185307179586f;
+    float diff = d.phi - t.p.y + 3.141592653589793f;
+    diff = diff - twopi * floorf(diff / twopi) - 3.141592653589793f;
+    e = t.p.x * diff * diff;
+    dE = 2.0f * t.p.x * diff;
+  } else {
+    const float a = t.p.x * d.phi - t.p.y;
+    float sa, ca;
+    sincosf(a, &sa, &ca);
+    e = t.p.z * (1.0f + ca);
+    dE = -t.p.z * t.p.x * sa;
+  }
+  scatter_torsion(v, t.idx, d, dE);
+  if (ENERGY) v.e_tors[i] = (double)e;
+}
+
+template <bool ENERGY>
+__device__ void do_cmap(const DeviceView& v, int i) {
+  const CmapEntry c = v.cmaps[i];
+  if (c.idx.x < 0) return;
+  const float3 p0 = ld3(v, c.idx.x), p1 = ld3(v, c.idx.y), p2 = ld3(v, c.idx.z), p3 = ld3(v, c.idx.w);
+  const float3 p4 = ld3(v, c.idx2.x);
+  const Dihedral dp = dihedral(p0, p1, p2, p3);
+  const Dihedral ds = dihedral(p1, p2, p3, p4);
+  const int G = c.idx2.z;
+  const float twopi = 6.283185307179586f;
+  const float delta = twopi / (float)G;
+  float ap = dp.phi + twopi;
+  ap = ap - twopi * floorf(ap / twopi);
+  float as = ds.phi + twopi;
+  as = as - twopi * floorf(as / twopi);
+  const float xa = ap / delta, xb = as / delta;
+  int ia = (int)floorf(xa), ib = (int)floorf(xb);
+  const float da = xa - (float)ia, db = xb - (float)ib;
+  ia = ((ia % G) + G) % G;
+  ib = ((ib % G) + G) % G;
+  const float* cf = v.cmap_coeff + c.idx2.y + ((size_t)ia * G + ib) * 16;
+  // E(a, b) = sum_p a^p R_p(b), R_p(b) = sum_q c[4p + q] b^q; Horner in a.
+  float e = 0.f, ea = 0.f, eb = 0.f;
+  for (int p = 3; p >= 0; --p) {
+    const float q0 = cf[4 * p], q1 = cf[4 * p + 1], q2 = cf[4 * p + 2], q3 = cf[4 * p + 3];
+    const float rv = ((q3 * db + q2) * db + q1) * db + q0;   // R_p(b)
+    const float rd = (3.0f * q3 * db + 2.0f * q2) * db + q1;  // R_p'(b)
+    ea = ea * da + e;  // d/da of the running Horner value
+    e = e * da + rv;
+    eb = eb * da + rd;
+  }
+  const float dEa = ea;
+  scatter_torsion(v, c.idx, dp, dEa / delta);
+  scatter_torsion(v, make_int4(c.idx.y, c.idx.z, c.idx.w, c.idx2.x), ds, eb / delta);
+  if (ENERGY) v.e_cmap[i] = (double)e;
+}
+
+template <bool ENERGY>
+__device__ void do_exception(const DeviceView& v, int i) {
+  const ExceptionEntry x = v.excs[i];
+  if (x.idx.x < 0) return;
+  const float3 d = sub3(ld3(v, x.idx.x), ld3(v, x.idx.y));
+  const float r2 = dot3(d, d) + 1e-12f;
+  const float r = sqrtf(r2);
+  const float s = x.p.x / r;
+  const float s2 = s * s;
+  const float s6 = s2 * s2 * s2;
+  const float e = 4.0f * x.p.y * (s6 * s6 - s6) + x.p.z / r;
+  // -dE/dr / r
+  const float fr = (4.0f * x.p.y * (12.0f * s6 * s6 - 6.0f * s6) + x.p.z / r) / r2;
+  scatter3(v, x.idx.x, fr * d.x, fr * d.y, fr * d.z);
+  scatter3(v, x.idx.y, -fr * d.x, -fr * d.y, -fr * d.z);
+  if (ENERGY) v.e_exc[i] = (double)e;
+}
+
+template <bool ENERGY>
+__global__ __launch_bounds__(kBondedThreads) void k_bonded(DeviceView v, int nb_bond, int nb_angle,
+                                                           int nb_tors, int nb_cmap) {
+  int b = blockIdx.x;
+  const int t = threadIdx.x;
+  if (b < nb_bond) {
+    const int i = b * kBondedThreads + t;
+    if (i < v.bond_hw) do_bond<ENERGY>(v, i);
+    return;
+  }
+  b -= nb_bond;
+  if (b < nb_angle) {
+    const int i = b * kBondedThreads + t;
+    if (i < v.angle_hw) do_angle<ENERGY>(v, i);
+    return;
+  }
+  b -= nb_angle;
+  if (b < nb_tors) {
+    const int i = b * kBondedThreads + t;
+    if (i < v.tors_hw) do_torsion<ENERGY>(v, i);
+    return;
+  }
+  b -= nb_tors;
+  if (b < nb_cmap) {
+    const int i = b * kBondedThreads + t;
+    if (i < v.cmap_hw) do_cmap<ENERGY>(v, i);
+    return;
+  }
+  b -= nb_cmap;
+  const int i = b * kBondedThreads + t;
+  if (i < v.exc_hw) do_exception<ENERGY>(v, i);
+}
+
+void launch_bonded(const DeviceView& v, bool energy, hipStream_t s) {
+  auto nb = [](int n) { return (n + kBondedThreads - 1) / kBondedThreads; };
+  const int b0 = nb(v.bond_hw), b1 = nb(v.angle_hw), b2 = nb(v.tors_hw), b3 = nb(v.cmap_hw), b4 = nb(v.exc_hw);
+  const int total = b0 + b1 + b2 + b3 + b4;
+  if (total == 0) return;
+  if (energy)
+    hipLaunchKernelGGL(k_bonded<true>, dim3(total), dim3(kBondedThreads), 0, s, v, b0, b1, b2, b3);
+  else
+    hipLaunchKernelGGL(k_bonded<false>, dim3(total), dim3(kBondedThreads), 0, s, v, b0, b1, b2, b3);
+}
+
+}  // namespace prx

@@ -1,0 +1,926 @@
+// Engine: device arenas, per-system upload, work list, captured BAOAB graphs
+// and the C ABI of include/prolix_gb.h.
+//
+// An engine holds any number of systems ("slots"). prx_system_create admits a
+// fixed set sized exactly; a pool admits and releases systems while others
+// keep stepping. Per-atom data lives in one arena (a system owns a contiguous
+// range), bonded entries in per-kind arenas, so admission never moves another
+// system's state.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "prolix_gb.h"
+#include "prx_common.h"
+#include "prx_host.h"
+
+namespace {
+
+thread_local std::string g_error;
+
+struct HipError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct CapacityError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct StateError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+#define HIPCHECK(x)                                                                          \
+  do {                                                                                       \
+    hipError_t e_ = (x);                                                                     \
+    if (e_ != hipSuccess)                                                                    \
+      throw HipError(std::string(#x) + ": " + hipGetErrorString(e_));                        \
+  } while (0)
+
+constexpr double kAkmaFs = 48.88821291839;  // 1 AKMA time unit in fs (simulate.py:56)
+constexpr double kBoltzmann = 0.0019872041; // kcal/mol/K (simulate.py:57)
+
+// First-fit allocator over [0, cap).
+class RangeAlloc {
+ public:
+  explicit RangeAlloc(int64_t cap = 0) : cap_(cap) { if (cap > 0) free_.push_back({0, cap}); }
+  int64_t alloc(int64_t n) {
+    if (n == 0) return 0;
+    for (size_t k = 0; k < free_.size(); ++k) {
+      if (free_[k].second - free_[k].first >= n) {
+        const int64_t b = free_[k].first;
+        free_[k].first += n;
+        if (free_[k].first == free_[k].second) free_.erase(free_.begin() + k);
+        return b;
+      }
+    }
+    return -1;
+  }
+  void release(int64_t b, int64_t n) {
+    if (n == 0) return;
+    free_.push_back({b, b + n});
+    std::sort(free_.begin(), free_.end());
+    std::vector<std::pair<int64_t, int64_t>> m;
+    for (auto& r : free_) {
+      if (!m.empty() && m.back().second == r.first) m.back().second = r.second;
+      else m.push_back(r);
+    }
+    free_ = m;
+  }
+  int64_t high_water() const {
+    // end of the last allocated byte: cap minus a free tail, if any
+    if (free_.empty()) return cap_;
+    if (free_.back().second == cap_) return free_.back().first;
+    return cap_;
+  }
+  int64_t cap() const { return cap_; }
+
+ private:
+  int64_t cap_;
+  std::vector<std::pair<int64_t, int64_t>> free_;
+};
+
+template <class T>
+T* dalloc(size_t n) {
+  if (n == 0) n = 1;
+  void* p = nullptr;
+  HIPCHECK(hipMalloc(&p, n * sizeof(T)));
+  HIPCHECK(hipMemset(p, 0, n * sizeof(T)));
+  return static_cast<T*>(p);
+}
+
+struct Slot {
+  bool live = false;
+  int32_t a0 = 0, n = 0;
+  int64_t bond_b = 0, bond_n = 0, ang_b = 0, ang_n = 0, tors_b = 0, tors_n = 0, cmap_b = 0, cmap_n = 0;
+  int64_t exc_b = 0, exc_n = 0, coeff_b = 0, coeff_n = 0, xt_b = 0, xt_n = 0;
+  std::vector<prx::PairItem> items;  // global indices, fixed for the slot's lifetime
+  prx::SlotConst c{};
+  bool has_params = false;
+};
+
+int64_t count_torsion_entries(const prx_system_desc& d) {
+  int64_t n = 0;
+  for (int t = 0; t < d.n_dihedrals; ++t)
+    for (int k = 0; k < d.dihedral_terms; ++k)
+      if (d.dihedral_params[((size_t)t * d.dihedral_terms + k) * 3 + 2] != 0.0) ++n;
+  const int w = d.improper_param_width;
+  for (int t = 0; t < d.n_impropers; ++t)
+    for (int k = 0; k < d.improper_terms; ++k) {
+      const double kk = d.improper_params[((size_t)t * d.improper_terms + k) * w + (w == 2 ? 0 : 2)];
+      if (kk != 0.0) ++n;
+    }
+  return n;
+}
+
+void check_idx(const int32_t* idx, int64_t rows, int cols, int n, const char* what) {
+  for (int64_t r = 0; r < rows * cols; ++r)
+    if (idx[r] < 0 || idx[r] >= n)
+      throw std::invalid_argument(std::string(what) + " index out of range");
+}
+
+void validate(const prx_system_desc& d) {
+  if (d.n_atoms <= 0) throw std::invalid_argument("n_atoms must be positive (charges is empty)");
+  if (!d.charges || !d.sigmas || !d.epsilons || !d.radii)
+    throw std::invalid_argument("charges, sigmas, epsilons and radii are required");
+  if (d.n_bonds) check_idx(d.bond_idx, d.n_bonds, 2, d.n_atoms, "bond");
+  if (d.n_ub) check_idx(d.ub_idx, d.n_ub, 2, d.n_atoms, "urey_bradley");
+  if (d.n_angles) check_idx(d.angle_idx, d.n_angles, 3, d.n_atoms, "angle");
+  if (d.n_dihedrals) {
+    if (d.dihedral_terms <= 0) throw std::invalid_argument("dihedral_terms must be positive");
+    check_idx(d.dihedral_idx, d.n_dihedrals, 4, d.n_atoms, "dihedral");
+  }
+  if (d.n_impropers) {
+    if (d.improper_terms <= 0) throw std::invalid_argument("improper_terms must be positive");
+    if (d.improper_param_width != 2 && d.improper_param_width != 3)
+      throw std::invalid_argument("improper_param_width must be 2 (harmonic) or 3 (periodic)");
+    check_idx(d.improper_idx, d.n_impropers, 4, d.n_atoms, "improper");
+  }
+  if (d.n_cmap) {
+    check_idx(d.cmap_torsions, d.n_cmap, 5, d.n_atoms, "cmap torsion");
+    if (d.cmap_grid < 2 || d.n_cmap_maps <= 0 || !d.cmap_grids)
+      throw std::invalid_argument("cmap needs grids (n_cmap_maps, G, G)");
+    for (int k = 0; k < d.n_cmap; ++k)
+      if (d.cmap_map[k] < 0 || d.cmap_map[k] >= d.n_cmap_maps) throw std::invalid_argument("cmap map index out of range");
+  }
+  if (d.n_exceptions) check_idx(d.exc_pairs, d.n_exceptions, 2, d.n_atoms, "exception");
+  if (d.cutoff < 0) throw std::invalid_argument("cutoff must be >= 0");
+  if (d.implicit_solvent) {
+    for (int i = 0; i < d.n_atoms; ++i)
+      if (!(d.radii[i] - d.dielectric_offset > 0.0))
+        throw std::invalid_argument("GB radii must exceed the dielectric offset");
+  }
+}
+
+}  // namespace
+
+struct prx_engine {
+  // capacities
+  int64_t cap_atoms = 0, cap_bond = 0, cap_ang = 0, cap_tors = 0, cap_cmap = 0, cap_exc = 0;
+  int64_t cap_coeff = 0, cap_xt = 0;
+  int32_t cap_slots = 0;
+  bool pool = false;
+  RangeAlloc a_atoms, a_bond, a_ang, a_tors, a_cmap, a_exc, a_coeff, a_xt;
+  std::vector<Slot> slots;
+  prx::DeviceView v{};
+  // device buffers owned here
+  prx::PairItem* d_items = nullptr;
+  int64_t cap_items = 0;
+  double* d_e_items = nullptr;
+  unsigned long long* d_excl = nullptr;
+  float* d_coeff = nullptr;
+  int32_t* d_ranges = nullptr;  // 14 ints per listed slot
+  int32_t* d_map = nullptr;     // packed atom -> arena index
+  int32_t* d_slot_list = nullptr;
+  double* d_energy_tmp = nullptr;
+  int64_t cap_map = 0;
+  std::vector<int32_t> order;   // live slots in packed order
+  std::vector<int32_t> offsets; // packed atom offsets of `order`
+  hipStream_t stream = nullptr;
+  bool dirty = true;            // work list / ranges need rebuilding
+  bool initialized = false;
+  std::vector<std::pair<int, hipGraphExec_t>> graphs;  // (steps per launch, executable)
+
+  void alloc_device() {
+    v.pos = dalloc<float4>(cap_atoms);
+    v.mom = dalloc<float4>(cap_atoms);
+    v.force = dalloc<float4>(cap_atoms);
+    v.born4 = dalloc<float4>(cap_atoms);
+    v.nb4 = dalloc<float4>(cap_atoms);
+    v.fin4 = dalloc<double4>(cap_atoms);
+    v.sqrt_m = dalloc<float>(cap_atoms);
+    v.slot_of = dalloc<int32_t>(cap_atoms);
+    HIPCHECK(hipMemset(v.slot_of, 0xFF, std::max<int64_t>(cap_atoms, 1) * sizeof(int32_t)));
+    v.gb4 = dalloc<float4>(cap_atoms);
+    v.acc = dalloc<unsigned long long>(cap_atoms * 4);
+    v.iacc = dalloc<unsigned long long>(cap_atoms);
+    v.e_atom = dalloc<double>(cap_atoms * 2);
+    v.atom_step = dalloc<uint32_t>(cap_atoms);
+    v.bonds = dalloc<prx::BondEntry>(cap_bond);
+    v.angles = dalloc<prx::AngleEntry>(cap_ang);
+    v.tors = dalloc<prx::TorsionEntry>(cap_tors);
+    v.cmaps = dalloc<prx::CmapEntry>(cap_cmap);
+    v.excs = dalloc<prx::ExceptionEntry>(cap_exc);
+    mark_free_entries();
+    v.e_bond = dalloc<double>(cap_bond);
+    v.e_angle = dalloc<double>(cap_ang);
+    v.e_tors = dalloc<double>(cap_tors);
+    v.e_cmap = dalloc<double>(cap_cmap);
+    v.e_exc = dalloc<double>(cap_exc);
+    d_coeff = dalloc<float>(cap_coeff);
+    v.cmap_coeff = d_coeff;
+    d_excl = dalloc<unsigned long long>(cap_xt * 128);
+    v.excl_masks = d_excl;
+    v.slots = dalloc<prx::SlotConst>(cap_slots);
+    v.max_slots = cap_slots;
+    d_ranges = dalloc<int32_t>((size_t)cap_slots * 14);
+    d_slot_list = dalloc<int32_t>(cap_slots);
+    d_energy_tmp = dalloc<double>((size_t)cap_slots * PRX_N_ENERGY);
+    cap_map = cap_atoms;
+    d_map = dalloc<int32_t>(cap_map);
+    HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  }
+
+  template <class E>
+  void free_entries(E* base, int64_t b, int64_t n) {
+    if (n <= 0) return;
+    std::vector<E> h((size_t)n);
+    for (auto& e : h) { std::memset(&e, 0, sizeof(E)); e.idx.x = -1; }
+    HIPCHECK(hipMemcpy(base + b, h.data(), n * sizeof(E), hipMemcpyHostToDevice));
+  }
+
+  void mark_free_entries() {
+    free_entries(v.bonds, 0, cap_bond);
+    free_entries(v.angles, 0, cap_ang);
+    free_entries(v.tors, 0, cap_tors);
+    free_entries(v.cmaps, 0, cap_cmap);
+    free_entries(v.excs, 0, cap_exc);
+  }
+
+  void release_device() {
+    drop_graphs();
+    void* ptrs[] = {v.pos, v.mom, v.force, v.born4, v.nb4, v.fin4, v.sqrt_m, v.slot_of, v.gb4, v.acc, v.iacc,
+                    v.e_atom, v.atom_step, v.bonds, v.angles, v.tors, v.cmaps, v.excs, v.e_bond, v.e_angle,
+                    v.e_tors, v.e_cmap, v.e_exc, d_coeff, d_excl, v.slots, d_ranges, d_slot_list,
+                    d_energy_tmp, d_map, d_items, d_e_items};
+    for (void* p : ptrs)
+      if (p) hipFree(p);
+    if (stream) hipStreamDestroy(stream);
+  }
+
+  // NULL is the device's legacy default stream (what torch's default stream
+  // is), never the engine's own non-blocking stream, so stream-ordered callers
+  // see their inputs and outputs ordered with our kernels.
+  hipStream_t pick(void* s) const { return (hipStream_t)s; }
+
+  // ---- admission -------------------------------------------------------
+  int32_t admit(const prx_system_desc& d) {
+    validate(d);
+    prx::ExclPlan plan;
+    prx::build_exclusions(d, plan);
+    int32_t sid = -1;
+    for (int32_t k = 0; k < (int32_t)slots.size(); ++k)
+      if (!slots[k].live) { sid = k; break; }
+    if (sid < 0) {
+      if ((int32_t)slots.size() >= cap_slots) throw CapacityError("no free system slot");
+      slots.emplace_back();
+      sid = (int32_t)slots.size() - 1;
+    }
+    Slot s;
+    s.n = d.n_atoms;
+    const int64_t n_bond = d.n_bonds + d.n_ub, n_ang = d.n_angles, n_tors = count_torsion_entries(d);
+    const int64_t n_cmap = d.n_cmap, n_exc = d.n_exceptions;
+    const int64_t n_coeff = d.n_cmap ? (int64_t)d.n_cmap_maps * d.cmap_grid * d.cmap_grid * 16 : 0;
+    const int64_t n_xt = (int64_t)plan.tiles.size();
+    struct Want { RangeAlloc* a; int64_t n; int64_t* b; };
+    int64_t a0 = 0;
+    Want w[] = {{&a_atoms, s.n, &a0}, {&a_bond, n_bond, &s.bond_b}, {&a_ang, n_ang, &s.ang_b},
+                {&a_tors, n_tors, &s.tors_b}, {&a_cmap, n_cmap, &s.cmap_b}, {&a_exc, n_exc, &s.exc_b},
+                {&a_coeff, n_coeff, &s.coeff_b}, {&a_xt, n_xt, &s.xt_b}};
+    int got = 0;
+    for (auto& x : w) {
+      const int64_t b = x.a->alloc(x.n);
+      if (b < 0) {
+        for (int k = 0; k < got; ++k) w[k].a->release(*w[k].b, w[k].n);
+        throw CapacityError("pool capacity exhausted");
+      }
+      *x.b = b;
+      ++got;
+    }
+    s.a0 = (int32_t)a0;
+    s.bond_n = n_bond; s.ang_n = n_ang; s.tors_n = n_tors; s.cmap_n = n_cmap; s.exc_n = n_exc;
+    s.coeff_n = n_coeff; s.xt_n = n_xt;
+    upload(d, sid, s, plan);
+    prx::build_items(s.n, s.a0, sid, plan, (int)s.xt_b, s.items);
+    s.live = true;
+    slots[sid] = std::move(s);
+    order.push_back(sid);
+    dirty = true;
+    return sid;
+  }
+
+  void upload(const prx_system_desc& d, int32_t sid, Slot& s, const prx::ExclPlan& plan) {
+    const int n = d.n_atoms;
+    const double off = d.dielectric_offset;
+    std::vector<float4> born4(n), nb4(n), mom(n, make_float4(0, 0, 0, 0));
+    std::vector<double4> fin4(n);
+    std::vector<float> sqm(n);
+    std::vector<int32_t> slot_of(n, sid);
+    for (int i = 0; i < n; ++i) {
+      const double R = d.radii[i];
+      const double sr = d.scaled_radii ? d.scaled_radii[i] : R;
+      const double rho = R - off;
+      const double q = d.charges[i];
+      const double sig = std::max(d.sigmas[i], 1e-6);
+      const double m = d.masses ? d.masses[i] : 1.0;
+      if (!(m > 0.0)) throw std::invalid_argument("masses must be positive");
+      born4[i] = make_float4((float)rho, (float)sr, (float)(sr * sr), 0.0f);
+      nb4[i] = make_float4((float)q, (float)(0.5 * sig), (float)std::sqrt(std::max(d.epsilons[i], 0.0)), 0.0f);
+      const double rn = (R - off) / 10.0 + 0.009;
+      const double aceK = d.implicit_solvent
+                              ? 0.239006 * 28.3919551 * (rn + 0.14) * (rn + 0.14) * std::pow(10.0 * rn, 6)
+                              : 0.0;
+      fin4[i] = make_double4(rho, R, q * q, aceK);
+      sqm[i] = (float)std::sqrt(m);
+      mom[i].w = (float)(1.0 / m);
+    }
+    HIPCHECK(hipMemcpy(v.born4 + s.a0, born4.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(v.nb4 + s.a0, nb4.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(v.fin4 + s.a0, fin4.data(), n * sizeof(double4), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(v.sqrt_m + s.a0, sqm.data(), n * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(v.mom + s.a0, mom.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(v.slot_of + s.a0, slot_of.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemset(v.atom_step + s.a0, 0, n * sizeof(uint32_t)));
+    HIPCHECK(hipMemset(v.pos + s.a0, 0, n * sizeof(float4)));
+    HIPCHECK(hipMemset(v.force + s.a0, 0, n * sizeof(float4)));
+
+    // bonded entries
+    const int a0 = s.a0;
+    if (s.bond_n) {
+      std::vector<prx::BondEntry> e((size_t)s.bond_n);
+      size_t k = 0;
+      for (int b = 0; b < d.n_bonds; ++b, ++k) {
+        e[k].idx = make_int4(a0 + d.bond_idx[2 * b], a0 + d.bond_idx[2 * b + 1], 0, 0);
+        e[k].p = make_float4((float)d.bond_params[2 * b], (float)d.bond_params[2 * b + 1], 0, 0);
+      }
+      for (int b = 0; b < d.n_ub; ++b, ++k) {
+        e[k].idx = make_int4(a0 + d.ub_idx[2 * b], a0 + d.ub_idx[2 * b + 1], 1, 0);
+        e[k].p = make_float4((float)d.ub_params[2 * b], (float)d.ub_params[2 * b + 1], 0, 0);
+      }
+      HIPCHECK(hipMemcpy(v.bonds + s.bond_b, e.data(), e.size() * sizeof(e[0]), hipMemcpyHostToDevice));
+    }
+    if (s.ang_n) {
+      std::vector<prx::AngleEntry> e((size_t)s.ang_n);
+      for (int b = 0; b < d.n_angles; ++b) {
+        e[b].idx = make_int4(a0 + d.angle_idx[3 * b], a0 + d.angle_idx[3 * b + 1], a0 + d.angle_idx[3 * b + 2], 0);
+        e[b].p = make_float4((float)d.angle_params[2 * b], (float)d.angle_params[2 * b + 1], 0, 0);
+      }
+      HIPCHECK(hipMemcpy(v.angles + s.ang_b, e.data(), e.size() * sizeof(e[0]), hipMemcpyHostToDevice));
+    }
+    if (s.tors_n) {
+      std::vector<prx::TorsionEntry> e;
+      e.reserve((size_t)s.tors_n);
+      for (int t = 0; t < d.n_dihedrals; ++t)
+        for (int k = 0; k < d.dihedral_terms; ++k) {
+          const double* p = &d.dihedral_params[((size_t)t * d.dihedral_terms + k) * 3];
+          if (p[2] == 0.0) continue;
+          prx::TorsionEntry x;
+          x.idx = make_int4(a0 + d.dihedral_idx[4 * t], a0 + d.dihedral_idx[4 * t + 1],
+                            a0 + d.dihedral_idx[4 * t + 2], a0 + d.dihedral_idx[4 * t + 3]);
+          x.p = make_float4((float)p[0], (float)p[1], (float)p[2], 0.0f);
+          e.push_back(x);
+        }
+      const int w = d.improper_param_width;
+      for (int t = 0; t < d.n_impropers; ++t)
+        for (int k = 0; k < d.improper_terms; ++k) {
+          const double* p = &d.improper_params[((size_t)t * d.improper_terms + k) * w];
+          prx::TorsionEntry x;
+          x.idx = make_int4(a0 + d.improper_idx[4 * t], a0 + d.improper_idx[4 * t + 1],
+                            a0 + d.improper_idx[4 * t + 2], a0 + d.improper_idx[4 * t + 3]);
+          if (w == 2) {
+            if (p[0] == 0.0) continue;
+            x.p = make_float4((float)p[0], (float)p[1], 0.0f, 2.0f);
+          } else {
+            if (p[2] == 0.0) continue;
+            x.p = make_float4((float)p[0], (float)p[1], (float)p[2], 1.0f);
+          }
+          e.push_back(x);
+        }
+      HIPCHECK(hipMemcpy(v.tors + s.tors_b, e.data(), e.size() * sizeof(e[0]), hipMemcpyHostToDevice));
+    }
+    if (s.cmap_n) {
+      const int G = d.cmap_grid;
+      std::vector<double> c64((size_t)G * G * 16);
+      std::vector<float> c32((size_t)s.coeff_n);
+      for (int m = 0; m < d.n_cmap_maps; ++m) {
+        prx::cmap_coefficients(d.cmap_grids + (size_t)m * G * G, G, c64.data());
+        for (size_t k = 0; k < c64.size(); ++k) c32[(size_t)m * G * G * 16 + k] = (float)c64[k];
+      }
+      HIPCHECK(hipMemcpy(d_coeff + s.coeff_b, c32.data(), c32.size() * sizeof(float), hipMemcpyHostToDevice));
+      std::vector<prx::CmapEntry> e((size_t)s.cmap_n);
+      for (int k = 0; k < d.n_cmap; ++k) {
+        const int32_t* t = &d.cmap_torsions[5 * k];
+        e[k].idx = make_int4(a0 + t[0], a0 + t[1], a0 + t[2], a0 + t[3]);
+        e[k].idx2 = make_int4(a0 + t[4], (int)(s.coeff_b + (int64_t)d.cmap_map[k] * G * G * 16), G, 0);
+      }
+      HIPCHECK(hipMemcpy(v.cmaps + s.cmap_b, e.data(), e.size() * sizeof(e[0]), hipMemcpyHostToDevice));
+    }
+    if (s.exc_n) {
+      std::vector<prx::ExceptionEntry> e((size_t)s.exc_n);
+      for (int k = 0; k < d.n_exceptions; ++k) {
+        e[k].idx = make_int4(a0 + d.exc_pairs[2 * k], a0 + d.exc_pairs[2 * k + 1], 0, 0);
+        e[k].p = make_float4((float)d.exc_sigma[k], (float)d.exc_epsilon[k],
+                             (float)(d.exc_chargeprod[k] * d.coulomb_constant), 0.0f);
+      }
+      HIPCHECK(hipMemcpy(v.excs + s.exc_b, e.data(), e.size() * sizeof(e[0]), hipMemcpyHostToDevice));
+    }
+    if (s.xt_n)
+      HIPCHECK(hipMemcpy(d_excl + s.xt_b * 128, plan.tile_masks.data(), plan.tile_masks.size() * 8,
+                         hipMemcpyHostToDevice));
+    // constants
+    prx::SlotConst& c = s.c;
+    std::memset(&c, 0, sizeof(c));
+    c.a0 = s.a0;
+    c.n = n;
+    c.active = 1;
+    c.cutoff2 = d.cutoff > 0 ? (float)(d.cutoff * d.cutoff) : 0.0f;
+    c.k_coulomb = (float)d.coulomb_constant;
+    const double pref = d.implicit_solvent
+                            ? -0.5 * d.gb_coulomb_constant * (1.0 / d.solute_dielectric - 1.0 / d.solvent_dielectric)
+                            : 0.0;
+    c.gb_pref = pref;
+    c.gb2 = (float)(2.0 * pref);
+    c.excl_v[0] = c.excl_e[0] = 1.0f;
+    for (size_t k = 0; k < plan.classes.size(); ++k) {
+      c.excl_v[k + 1] = (float)plan.classes[k].first;
+      c.excl_e[k + 1] = (float)plan.classes[k].second;
+    }
+    push_const(sid, c);
+  }
+
+  void push_const(int32_t sid, const prx::SlotConst& c) {
+    HIPCHECK(hipMemcpy(v.slots + sid, &c, sizeof(c), hipMemcpyHostToDevice));
+  }
+
+  void set_params(int32_t sid, const prx_langevin_params& p) {
+    if (!(p.dt_fs > 0.0) || p.temperature < 0.0 || p.gamma_ps < 0.0)
+      throw std::invalid_argument("integrator needs dt_fs > 0, temperature >= 0, gamma_ps >= 0");
+    prx::SlotConst& c = slots[sid].c;
+    const double dt = p.dt_fs / kAkmaFs;
+    const double gamma = p.gamma_ps * kAkmaFs * 1e-3;
+    const double kT = kBoltzmann * p.temperature;
+    const double c1 = std::exp(-gamma * dt);
+    c.dt = (float)dt;
+    c.half_dt = (float)(0.5 * dt);
+    c.c1 = (float)c1;
+    c.c2 = (float)std::sqrt(std::max(0.0, 1.0 - c1 * c1));
+    c.kT = (float)kT;
+    c.sqrt_kT = (float)std::sqrt(kT);
+    c.seed = p.seed;
+    slots[sid].has_params = true;
+    push_const(sid, c);
+  }
+
+  void release(int32_t sid) {
+    if (sid < 0 || sid >= (int32_t)slots.size() || !slots[sid].live) throw std::invalid_argument("no such live slot");
+    Slot& s = slots[sid];
+    HIPCHECK(hipMemset(v.slot_of + s.a0, 0xFF, s.n * sizeof(int32_t)));
+    free_entries(v.bonds, s.bond_b, s.bond_n);
+    free_entries(v.angles, s.ang_b, s.ang_n);
+    free_entries(v.tors, s.tors_b, s.tors_n);
+    free_entries(v.cmaps, s.cmap_b, s.cmap_n);
+    free_entries(v.excs, s.exc_b, s.exc_n);
+    a_atoms.release(s.a0, s.n);
+    a_bond.release(s.bond_b, s.bond_n);
+    a_ang.release(s.ang_b, s.ang_n);
+    a_tors.release(s.tors_b, s.tors_n);
+    a_cmap.release(s.cmap_b, s.cmap_n);
+    a_exc.release(s.exc_b, s.exc_n);
+    a_coeff.release(s.coeff_b, s.coeff_n);
+    a_xt.release(s.xt_b, s.xt_n);
+    s = Slot();
+    order.erase(std::remove(order.begin(), order.end(), sid), order.end());
+    dirty = true;
+  }
+
+  // ---- work list and ranges ---------------------------------------------
+  void rebuild() {
+    if (!dirty) return;
+    std::vector<prx::PairItem> items;
+    std::vector<int32_t> ranges;
+    std::vector<int32_t> map;
+    offsets.assign(1, 0);
+    for (int32_t sid : order) {
+      const Slot& s = slots[sid];
+      const int32_t ib = (int32_t)items.size();
+      items.insert(items.end(), s.items.begin(), s.items.end());
+      const int32_t r[14] = {ib, (int32_t)items.size(), s.a0, s.n, (int32_t)s.bond_b, (int32_t)(s.bond_b + s.bond_n),
+                             (int32_t)s.ang_b, (int32_t)(s.ang_b + s.ang_n), (int32_t)s.tors_b,
+                             (int32_t)(s.tors_b + s.tors_n), (int32_t)s.cmap_b, (int32_t)(s.cmap_b + s.cmap_n),
+                             (int32_t)s.exc_b, (int32_t)(s.exc_b + s.exc_n)};
+      ranges.insert(ranges.end(), r, r + 14);
+      for (int i = 0; i < s.n; ++i) map.push_back(s.a0 + i);
+      offsets.push_back(offsets.back() + s.n);
+    }
+    if ((int64_t)items.size() > cap_items) {
+      if (d_items) HIPCHECK(hipFree(d_items));
+      if (d_e_items) HIPCHECK(hipFree(d_e_items));
+      cap_items = std::max<int64_t>((int64_t)items.size(), cap_items * 3 / 2);
+      d_items = dalloc<prx::PairItem>(cap_items);
+      d_e_items = dalloc<double>(cap_items * 3);
+    }
+    if (!items.empty())
+      HIPCHECK(hipMemcpy(d_items, items.data(), items.size() * sizeof(prx::PairItem), hipMemcpyHostToDevice));
+    if (!ranges.empty())
+      HIPCHECK(hipMemcpy(d_ranges, ranges.data(), ranges.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    if (!map.empty()) HIPCHECK(hipMemcpy(d_map, map.data(), map.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    v.items = d_items;
+    v.n_items = (int32_t)items.size();
+    v.e_items = d_e_items;
+    v.atom_hw = (int32_t)a_atoms.high_water();
+    v.bond_hw = (int32_t)a_bond.high_water();
+    v.angle_hw = (int32_t)a_ang.high_water();
+    v.tors_hw = (int32_t)a_tors.high_water();
+    v.cmap_hw = (int32_t)a_cmap.high_water();
+    v.exc_hw = (int32_t)a_exc.high_water();
+    drop_graphs();
+    dirty = false;
+  }
+
+  int32_t total_atoms() const { return offsets.back(); }
+
+  // ---- evaluation -------------------------------------------------------
+  void force_passes(bool energy, hipStream_t s) {
+    prx::launch_born_pairs(v, s);
+    prx::launch_born_finish(v, s);
+    prx::launch_force_pairs(v, energy, s);
+    prx::launch_chain_prep(v, energy, s);
+    prx::launch_chain_pairs(v, s);
+    prx::launch_bonded(v, energy, s);
+  }
+
+  void evaluate(bool energy, hipStream_t s) {
+    prx::launch_zero_acc(v, s);
+    force_passes(energy, s);
+    prx::launch_gather_forces(v, s);
+  }
+
+  void drop_graphs() {
+    for (auto& g : graphs) hipGraphExecDestroy(g.second);
+    graphs.clear();
+  }
+
+  hipGraphExec_t capture(int n_steps) {
+    for (auto& g : graphs)
+      if (g.first == n_steps) return g.second;
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t g;
+    HIPCHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
+    prx::launch_kick_drift(v, 0, stream);
+    for (int k = 0; k < n_steps; ++k) {
+      force_passes(false, stream);
+      prx::launch_kick_drift(v, k + 1 == n_steps ? 2 : 1, stream);
+    }
+    HIPCHECK(hipStreamEndCapture(stream, &g));
+    HIPCHECK(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
+    HIPCHECK(hipGraphDestroy(g));
+    graphs.push_back({n_steps, exec});
+    return exec;
+  }
+
+  void steps(int n_steps, hipStream_t s) {
+    rebuild();
+    if (order.empty() || n_steps <= 0) return;
+    // Graphs are captured on the engine stream; order them after the caller's stream.
+    const int chunk = std::min(n_steps, 50);
+    hipEvent_t ev;
+    HIPCHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    if (s != stream) {
+      HIPCHECK(hipEventRecord(ev, s));
+      HIPCHECK(hipStreamWaitEvent(stream, ev, 0));
+    }
+    int left = n_steps;
+    while (left > 0) {
+      const int k = std::min(chunk, left);
+      HIPCHECK(hipGraphLaunch(capture(k), stream));
+      left -= k;
+    }
+    if (s != stream) {
+      HIPCHECK(hipEventRecord(ev, stream));
+      HIPCHECK(hipStreamWaitEvent(s, ev, 0));
+    }
+    HIPCHECK(hipEventDestroy(ev));
+  }
+};
+
+namespace {
+
+template <class F>
+int guard(F&& f) {
+  try {
+    f();
+    g_error.clear();
+    return PRX_OK;
+  } catch (const CapacityError& e) {
+    g_error = e.what();
+    return PRX_ERR_CAPACITY;
+  } catch (const StateError& e) {
+    g_error = e.what();
+    return PRX_ERR_STATE;
+  } catch (const HipError& e) {
+    g_error = e.what();
+    return PRX_ERR_HIP;
+  } catch (const std::invalid_argument& e) {
+    g_error = e.what();
+    return PRX_ERR_INVALID;
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return PRX_ERR_INVALID;
+  }
+}
+
+void need(bool c, const char* msg) {
+  if (!c) throw std::invalid_argument(msg);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* prx_last_error(void) { return g_error.c_str(); }
+int32_t prx_abi_version(void) { return PRX_ABI_VERSION; }
+
+int prx_system_create(const prx_system_desc* descs, int32_t n_systems, prx_engine** out) {
+  return guard([&] {
+    need(descs != nullptr && out != nullptr && n_systems > 0, "prx_system_create: bad arguments");
+    *out = nullptr;
+    auto eng = std::make_unique<prx_engine>();
+    for (int k = 0; k < n_systems; ++k) {
+      const prx_system_desc& d = descs[k];
+      validate(d);
+      prx::ExclPlan plan;
+      prx::build_exclusions(d, plan);
+      eng->cap_atoms += d.n_atoms;
+      eng->cap_bond += d.n_bonds + d.n_ub;
+      eng->cap_ang += d.n_angles;
+      eng->cap_tors += count_torsion_entries(d);
+      eng->cap_cmap += d.n_cmap;
+      eng->cap_exc += d.n_exceptions;
+      eng->cap_coeff += d.n_cmap ? (int64_t)d.n_cmap_maps * d.cmap_grid * d.cmap_grid * 16 : 0;
+      eng->cap_xt += (int64_t)plan.tiles.size();
+    }
+    eng->cap_slots = n_systems;
+    eng->a_atoms = RangeAlloc(eng->cap_atoms);
+    eng->a_bond = RangeAlloc(eng->cap_bond);
+    eng->a_ang = RangeAlloc(eng->cap_ang);
+    eng->a_tors = RangeAlloc(eng->cap_tors);
+    eng->a_cmap = RangeAlloc(eng->cap_cmap);
+    eng->a_exc = RangeAlloc(eng->cap_exc);
+    eng->a_coeff = RangeAlloc(eng->cap_coeff);
+    eng->a_xt = RangeAlloc(eng->cap_xt);
+    try {
+      eng->alloc_device();
+      for (int k = 0; k < n_systems; ++k) eng->admit(descs[k]);
+      eng->rebuild();
+    } catch (...) {
+      eng->release_device();
+      throw;
+    }
+    *out = eng.release();
+  });
+}
+
+int prx_pool_create(const prx_pool_limits* lim, prx_engine** out) {
+  return guard([&] {
+    need(lim != nullptr && out != nullptr, "prx_pool_create: bad arguments");
+    need(lim->max_atoms > 0 && lim->max_systems > 0 && lim->max_systems < 65536,
+         "prx_pool_create: max_atoms > 0 and 0 < max_systems < 65536 required");
+    *out = nullptr;
+    auto eng = std::make_unique<prx_engine>();
+    eng->pool = true;
+    eng->cap_atoms = lim->max_atoms;
+    eng->cap_bond = std::max<int64_t>(lim->max_bonded, 1);
+    eng->cap_ang = eng->cap_bond;
+    eng->cap_tors = eng->cap_bond;
+    eng->cap_cmap = std::max<int64_t>(lim->max_bonded / 8, 1);
+    eng->cap_exc = std::max<int64_t>(lim->max_bonded / 8, 1);
+    eng->cap_coeff = std::max<int64_t>(lim->max_cmap_coeff, 1);
+    eng->cap_xt = std::max<int64_t>(lim->max_excl_tiles, 1);
+    eng->cap_slots = lim->max_systems;
+    eng->a_atoms = RangeAlloc(eng->cap_atoms);
+    eng->a_bond = RangeAlloc(eng->cap_bond);
+    eng->a_ang = RangeAlloc(eng->cap_ang);
+    eng->a_tors = RangeAlloc(eng->cap_tors);
+    eng->a_cmap = RangeAlloc(eng->cap_cmap);
+    eng->a_exc = RangeAlloc(eng->cap_exc);
+    eng->a_coeff = RangeAlloc(eng->cap_coeff);
+    eng->a_xt = RangeAlloc(eng->cap_xt);
+    try {
+      eng->alloc_device();
+    } catch (...) {
+      eng->release_device();
+      throw;
+    }
+    eng->initialized = true;
+    *out = eng.release();
+  });
+}
+
+int prx_engine_destroy(prx_engine* eng) {
+  return guard([&] {
+    if (!eng) return;
+    hipStreamSynchronize(eng->stream);
+    eng->release_device();
+    delete eng;
+  });
+}
+
+int prx_engine_layout(const prx_engine* eng, int32_t* offsets, int32_t capacity) {
+  return guard([&] {
+    need(eng && offsets, "prx_engine_layout: bad arguments");
+    need(capacity >= (int32_t)eng->offsets.size(), "prx_engine_layout: capacity too small");
+    std::copy(eng->offsets.begin(), eng->offsets.end(), offsets);
+  });
+}
+
+int prx_energy_forces(prx_engine* eng, const float* positions, float* forces, double* energies,
+                      float* born_radii, void* stream) {
+  return guard([&] {
+    need(eng && positions, "prx_energy_forces: bad arguments");
+    eng->rebuild();
+    hipStream_t s = eng->pick(stream);
+    const int n = eng->total_atoms();
+    prx::launch_copy_in_positions(eng->v, positions, eng->d_map, n, s);
+    eng->evaluate(energies != nullptr, s);
+    if (energies)
+      prx::launch_energy_reduce(eng->v, eng->d_ranges, (int)eng->order.size(), energies, s);
+    if (forces) prx::launch_copy_out(eng->v, nullptr, nullptr, forces, eng->d_map, n, s);
+    if (born_radii) prx::launch_born_out(eng->v, born_radii, eng->d_map, n, s);
+    HIPCHECK(hipGetLastError());
+  });
+}
+
+int prx_integrator_init(prx_engine* eng, const float* positions, const float* momenta,
+                        const prx_langevin_params* params, void* stream) {
+  return guard([&] {
+    need(eng && positions && params, "prx_integrator_init: bad arguments");
+    eng->rebuild();
+    hipStream_t s = eng->pick(stream);
+    const int n = eng->total_atoms();
+    for (size_t k = 0; k < eng->order.size(); ++k) eng->set_params(eng->order[k], params[k]);
+    prx::launch_copy_in_positions(eng->v, positions, eng->d_map, n, s);
+    if (momenta) {
+      prx::launch_copy_in_momenta(eng->v, momenta, eng->d_map, n, s);
+      HIPCHECK(hipMemsetAsync(eng->v.atom_step, 0, eng->cap_atoms * sizeof(uint32_t), s));
+    } else {
+      HIPCHECK(hipMemcpyAsync(eng->d_slot_list, eng->order.data(), eng->order.size() * sizeof(int32_t),
+                              hipMemcpyHostToDevice, s));
+      prx::launch_init_momenta(eng->v, eng->d_slot_list, (int)eng->order.size(), s);
+    }
+    eng->evaluate(false, s);
+    HIPCHECK(hipGetLastError());
+    eng->initialized = true;
+  });
+}
+
+int prx_integrator_steps(prx_engine* eng, int32_t n_steps, void* stream) {
+  return guard([&] {
+    need(eng != nullptr && n_steps >= 0, "prx_integrator_steps: bad arguments");
+    if (!eng->initialized) throw StateError("prx_integrator_steps before prx_integrator_init");
+    for (int32_t sid : eng->order)
+      if (!eng->slots[sid].has_params) throw StateError("a system has no integrator parameters");
+    eng->steps(n_steps, eng->pick(stream));
+    HIPCHECK(hipGetLastError());
+  });
+}
+
+int prx_integrator_state(prx_engine* eng, float* positions, float* momenta, float* forces, void* stream) {
+  return guard([&] {
+    need(eng != nullptr, "prx_integrator_state: bad arguments");
+    eng->rebuild();
+    prx::launch_copy_out(eng->v, positions, momenta, forces, eng->d_map, eng->total_atoms(), eng->pick(stream));
+    HIPCHECK(hipGetLastError());
+  });
+}
+
+int prx_integrator_kinetic(prx_engine* eng, double* kinetic, void* stream) {
+  return guard([&] {
+    need(eng && kinetic, "prx_integrator_kinetic: bad arguments");
+    eng->rebuild();
+    prx::launch_kinetic(eng->v, eng->d_ranges, (int)eng->order.size(), kinetic, eng->pick(stream));
+    HIPCHECK(hipGetLastError());
+  });
+}
+
+int prx_integrator_step_count(prx_engine* eng, int32_t system, int64_t* steps) {
+  return guard([&] {
+    need(eng && steps && system >= 0 && system < (int32_t)eng->order.size(), "prx_integrator_step_count: bad arguments");
+    const Slot& s = eng->slots[eng->order[system]];
+    uint32_t c = 0;
+    HIPCHECK(hipStreamSynchronize(eng->stream));
+    HIPCHECK(hipDeviceSynchronize());
+    HIPCHECK(hipMemcpy(&c, eng->v.atom_step + s.a0, sizeof(c), hipMemcpyDeviceToHost));
+    *steps = c;
+  });
+}
+
+int prx_pool_admit(prx_engine* pool, const prx_system_desc* desc, const float* positions_host,
+                   const prx_langevin_params* params, int32_t* slot, void* stream) {
+  return guard([&] {
+    need(pool && pool->pool && desc && positions_host && params && slot, "prx_pool_admit: bad arguments");
+    hipStream_t s = pool->pick(stream);
+    HIPCHECK(hipStreamSynchronize(s));
+    HIPCHECK(hipStreamSynchronize(pool->stream));
+    const int32_t sid = pool->admit(*desc);
+    try {
+      pool->set_params(sid, *params);
+      const Slot& sl = pool->slots[sid];
+      std::vector<float4> p4(sl.n);
+      for (int i = 0; i < sl.n; ++i)
+        p4[i] = make_float4(positions_host[3 * i], positions_host[3 * i + 1], positions_host[3 * i + 2], 0.f);
+      HIPCHECK(hipMemcpy(pool->v.pos + sl.a0, p4.data(), sl.n * sizeof(float4), hipMemcpyHostToDevice));
+      pool->rebuild();
+      HIPCHECK(hipMemcpy(pool->d_slot_list, &sid, sizeof(int32_t), hipMemcpyHostToDevice));
+      prx::launch_init_momenta(pool->v, pool->d_slot_list, 1, s);
+      // Forces for every live system at its current positions (deterministic, so
+      // the others' stored forces are reproduced bit for bit).
+      pool->evaluate(false, s);
+      HIPCHECK(hipGetLastError());
+      HIPCHECK(hipStreamSynchronize(s));
+    } catch (...) {
+      pool->release(sid);
+      throw;
+    }
+    *slot = sid;
+  });
+}
+
+int prx_pool_release(prx_engine* pool, int32_t slot, void* stream) {
+  return guard([&] {
+    need(pool && pool->pool, "prx_pool_release: bad arguments");
+    HIPCHECK(hipStreamSynchronize(pool->pick(stream)));
+    HIPCHECK(hipStreamSynchronize(pool->stream));
+    pool->release(slot);
+  });
+}
+
+int prx_pool_read(prx_engine* pool, int32_t slot, float* positions, float* momenta, float* forces,
+                  int64_t* steps, void* stream) {
+  return guard([&] {
+    need(pool && slot >= 0 && slot < (int32_t)pool->slots.size() && pool->slots[slot].live,
+         "prx_pool_read: no such live slot");
+    hipStream_t s = pool->pick(stream);
+    HIPCHECK(hipStreamSynchronize(s));
+    HIPCHECK(hipStreamSynchronize(pool->stream));
+    const Slot& sl = pool->slots[slot];
+    std::vector<float4> tmp(sl.n);
+    auto get = [&](float4* src, float* dst) {
+      if (!dst) return;
+      HIPCHECK(hipMemcpy(tmp.data(), src + sl.a0, sl.n * sizeof(float4), hipMemcpyDeviceToHost));
+      for (int i = 0; i < sl.n; ++i) { dst[3 * i] = tmp[i].x; dst[3 * i + 1] = tmp[i].y; dst[3 * i + 2] = tmp[i].z; }
+    };
+    get(pool->v.pos, positions);
+    get(pool->v.mom, momenta);
+    get(pool->v.force, forces);
+    if (steps) {
+      uint32_t c = 0;
+      HIPCHECK(hipMemcpy(&c, pool->v.atom_step + sl.a0, sizeof(c), hipMemcpyDeviceToHost));
+      *steps = c;
+    }
+  });
+}
+
+int prx_pool_slots(const prx_engine* pool, int32_t* slots, int32_t capacity, int32_t* n_live) {
+  return guard([&] {
+    need(pool && n_live, "prx_pool_slots: bad arguments");
+    *n_live = (int32_t)pool->order.size();
+    if (slots) {
+      need(capacity >= (int32_t)pool->order.size(), "prx_pool_slots: capacity too small");
+      std::copy(pool->order.begin(), pool->order.end(), slots);
+    }
+  });
+}
+
+int prx_pool_energy(prx_engine* pool, int32_t slot, double* energies_host, void* stream) {
+  return guard([&] {
+    need(pool && energies_host && slot >= 0 && slot < (int32_t)pool->slots.size() && pool->slots[slot].live,
+         "prx_pool_energy: no such live slot");
+    pool->rebuild();
+    hipStream_t s = pool->pick(stream);
+    HIPCHECK(hipStreamSynchronize(pool->stream));
+    pool->evaluate(true, s);
+    prx::launch_energy_reduce(pool->v, pool->d_ranges, (int)pool->order.size(), pool->d_energy_tmp, s);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(s));
+    const auto it = std::find(pool->order.begin(), pool->order.end(), slot);
+    const size_t row = (size_t)(it - pool->order.begin());
+    HIPCHECK(hipMemcpy(energies_host, pool->d_energy_tmp + row * PRX_N_ENERGY, PRX_N_ENERGY * sizeof(double),
+                       hipMemcpyDeviceToHost));
+  });
+}
+
+int prx_cmap_coefficients(const double* grid, int32_t G, double* out) {
+  return guard([&] {
+    need(grid && out && G >= 2, "prx_cmap_coefficients: bad arguments");
+    prx::cmap_coefficients(grid, G, out);
+  });
+}
+
+int prx_plan_tiles(const prx_system_desc* desc, int64_t* n_items, int64_t* n_excl_tiles) {
+  return guard([&] {
+    need(desc && n_items && n_excl_tiles, "prx_plan_tiles: bad arguments");
+    validate(*desc);
+    prx::ExclPlan plan;
+    prx::build_exclusions(*desc, plan);
+    const int64_t nT = (desc->n_atoms + prx::kTile - 1) / prx::kTile;
+    *n_items = nT * (nT + 1) / 2;
+    *n_excl_tiles = (int64_t)plan.tiles.size();
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,237 @@
+// Host-side planning that needs no GPU: CMAP patch coefficients, exclusion
+// classes and tile masks, and the pair work list of one system.
+#include "prx_host.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <stdexcept>
+
+namespace prx {
+
+// Solve A x = b (n x n, row-major) by Gaussian elimination with partial pivoting.
+static void dense_solve(std::vector<double>& A, std::vector<double>& b, int n) {
+  for (int c = 0; c < n; ++c) {
+    int piv = c;
+    for (int r = c + 1; r < n; ++r)
+      if (std::fabs(A[r * n + c]) > std::fabs(A[piv * n + c])) piv = r;
+    if (A[piv * n + c] == 0.0) throw std::runtime_error("singular system in CMAP spline");
+    if (piv != c) {
+      for (int k = 0; k < n; ++k) std::swap(A[c * n + k], A[piv * n + k]);
+      std::swap(b[c], b[piv]);
+    }
+    for (int r = c + 1; r < n; ++r) {
+      const double f = A[r * n + c] / A[c * n + c];
+      if (f == 0.0) continue;
+      for (int k = c; k < n; ++k) A[r * n + k] -= f * A[c * n + k];
+      b[r] -= f * b[c];
+    }
+  }
+  for (int r = n - 1; r >= 0; --r) {
+    double s = b[r];
+    for (int k = r + 1; k < n; ++k) s -= A[r * n + k] * b[k];
+    b[r] = s / A[r * n + r];
+  }
+}
+
+// Inverse of the 16 x 16 bicubic constraint matrix: rows are the values,
+// d/da, d/db and d2/dadb at corners (0,0), (1,0), (1,1), (0,1); columns the
+// coefficients c[4p + q] of sum c a^p b^q.
+static const std::vector<double>& bicubic_inverse() {
+  static std::vector<double> inv;
+  if (!inv.empty()) return inv;
+  const int corners[4][2] = {{0, 0}, {1, 0}, {1, 1}, {0, 1}};
+  std::vector<double> M(256, 0.0);
+  for (int kind = 0; kind < 4; ++kind)
+    for (int ci = 0; ci < 4; ++ci) {
+      const double a = corners[ci][0], b = corners[ci][1];
+      for (int p = 0; p < 4; ++p)
+        for (int q = 0; q < 4; ++q) {
+          double v = 0.0;
+          const double ap = std::pow(a, p), bq = std::pow(b, q);
+          const double apm = p > 0 ? p * std::pow(a, p - 1) : 0.0;
+          const double bqm = q > 0 ? q * std::pow(b, q - 1) : 0.0;
+          if (kind == 0) v = ap * bq;
+          else if (kind == 1) v = apm * bq;
+          else if (kind == 2) v = ap * bqm;
+          else v = apm * bqm;
+          M[(kind * 4 + ci) * 16 + 4 * p + q] = v;
+        }
+    }
+  inv.assign(256, 0.0);
+  for (int col = 0; col < 16; ++col) {
+    std::vector<double> A = M, e(16, 0.0);
+    e[col] = 1.0;
+    dense_solve(A, e, 16);
+    for (int r = 0; r < 16; ++r) inv[r * 16 + col] = e[r];
+  }
+  return inv;
+}
+
+// Periodic cubic spline derivatives at the nodes x_k = 2 pi k / G of samples y
+// (cmap.py:62-163): the cyclic system of cmap.py:84-121, evaluated at t = x_i.
+static void spline_node_derivs(const double* y, int G, double* dy) {
+  const int n = G;
+  std::vector<double> x(n + 1), h(n), yy(n + 1);
+  for (int k = 0; k <= n; ++k) x[k] = 2.0 * M_PI * k / n;
+  for (int k = 0; k < n; ++k) yy[k] = y[k];
+  yy[n] = y[0];
+  for (int k = 0; k < n; ++k) h[k] = x[k + 1] - x[k];
+  std::vector<double> slope(n), b(n), A((size_t)n * n, 0.0);
+  for (int k = 0; k < n; ++k) slope[k] = (yy[k + 1] - yy[k]) / h[k];
+  b[0] = 6.0 * (slope[0] - slope[n - 1]);
+  for (int k = 1; k < n; ++k) b[k] = 6.0 * (slope[k] - slope[k - 1]);
+  for (int k = 0; k < n; ++k) A[(size_t)k * n + k] = 2.0 * (h[(k - 1 + n) % n] + h[k]);
+  for (int k = 1; k < n; ++k) A[(size_t)k * n + k - 1] = h[k];          // lower = h[1:]
+  for (int k = 0; k + 1 < n; ++k) A[(size_t)k * n + k + 1] = h[(k + 1) % n];  // upper = roll(h, -1)[:-1]
+  A[0 * n + (n - 1)] = h[n - 1];
+  A[(size_t)(n - 1) * n + 0] = h[n - 1];
+  dense_solve(A, b, n);
+  std::vector<double> m(n + 1);
+  for (int k = 0; k < n; ++k) m[k] = b[k];
+  m[n] = b[0];
+  for (int i = 0; i < n; ++i) {
+    const double hi = x[i + 1] - x[i];
+    dy[i] = (yy[i + 1] - yy[i]) / hi + hi / 6.0 * (-2.0 * m[i] - m[i + 1]);
+  }
+}
+
+void cmap_coefficients(const double* grid_psi_phi, int G, double* out) {
+  // E[phi][psi] = grid[psi][phi] (cmap.py:259)
+  std::vector<double> E((size_t)G * G), d1(E.size()), d2(E.size()), d12(E.size());
+  for (int i = 0; i < G; ++i)
+    for (int j = 0; j < G; ++j) E[(size_t)i * G + j] = grid_psi_phi[(size_t)j * G + i];
+  std::vector<double> col(G), dcol(G);
+  // d1: along phi (axis 0) for each psi column
+  for (int j = 0; j < G; ++j) {
+    for (int i = 0; i < G; ++i) col[i] = E[(size_t)i * G + j];
+    spline_node_derivs(col.data(), G, dcol.data());
+    for (int i = 0; i < G; ++i) d1[(size_t)i * G + j] = dcol[i];
+  }
+  // d2: along psi (axis 1) for each phi row
+  for (int i = 0; i < G; ++i) {
+    spline_node_derivs(&E[(size_t)i * G], G, dcol.data());
+    for (int j = 0; j < G; ++j) d2[(size_t)i * G + j] = dcol[j];
+  }
+  // d12: along phi of d2 (cmap.py:204-214)
+  for (int j = 0; j < G; ++j) {
+    for (int i = 0; i < G; ++i) col[i] = d2[(size_t)i * G + j];
+    spline_node_derivs(col.data(), G, dcol.data());
+    for (int i = 0; i < G; ++i) d12[(size_t)i * G + j] = dcol[i];
+  }
+  const std::vector<double>& W = bicubic_inverse();
+  const double delta = 2.0 * M_PI / G;
+  for (int i = 0; i < G; ++i)
+    for (int j = 0; j < G; ++j) {
+      const int ni = (i + 1) % G, nj = (j + 1) % G;
+      const int ca[4] = {i, ni, ni, i}, cb[4] = {j, j, nj, nj};
+      double rhs[16];
+      for (int c = 0; c < 4; ++c) {
+        const size_t k = (size_t)ca[c] * G + cb[c];
+        rhs[c] = E[k];
+        rhs[4 + c] = d1[k] * delta;
+        rhs[8 + c] = d2[k] * delta;
+        rhs[12 + c] = d12[k] * delta * delta;
+      }
+      double* o = out + ((size_t)i * G + j) * 16;
+      for (int r = 0; r < 16; ++r) {
+        double s = 0.0;
+        for (int c = 0; c < 16; ++c) s += W[r * 16 + c] * rhs[c];
+        o[r] = s;
+      }
+    }
+}
+
+void build_exclusions(const prx_system_desc& d, ExclPlan& plan) {
+  plan.classes.clear();
+  plan.tiles.clear();
+  plan.tile_masks.clear();
+  const int n = d.n_atoms, C = d.excl_cols;
+  if (C <= 0 || d.excl_idx == nullptr) return;
+  // S(i, j) from row i: sum of matching entries' scales (optimization.py:55-57).
+  std::map<std::pair<int, int>, std::pair<double, double>> row;
+  for (int i = 0; i < n; ++i)
+    for (int k = 0; k < C; ++k) {
+      const int j = d.excl_idx[(size_t)i * C + k];
+      if (j < 0) continue;
+      if (j >= n) throw std::invalid_argument("exclusion index out of range");
+      if (j == i) continue;
+      auto& e = row[{i, j}];
+      e.first += d.excl_scale_vdw ? d.excl_scale_vdw[(size_t)i * C + k] : 0.0;
+      e.second += d.excl_scale_elec ? d.excl_scale_elec[(size_t)i * C + k] : 0.0;
+    }
+  // Symmetrise: the unordered pair carries (S(i,j) + S(j,i)) / 2 of each scale,
+  // which is the reference's 1/2 sum over both ordered pairs.
+  std::map<std::pair<int, int>, std::pair<double, double>> pairs;
+  for (const auto& kv : row) {
+    const int i = kv.first.first, j = kv.first.second;
+    const int a = std::min(i, j), b = std::max(i, j);
+    if (pairs.count({a, b})) continue;
+    auto it1 = row.find({a, b}), it2 = row.find({b, a});
+    const double v1 = it1 != row.end() ? it1->second.first : 1.0, e1 = it1 != row.end() ? it1->second.second : 1.0;
+    const double v2 = it2 != row.end() ? it2->second.first : 1.0, e2 = it2 != row.end() ? it2->second.second : 1.0;
+    pairs[{a, b}] = {0.5 * (v1 + v2), 0.5 * (e1 + e2)};
+  }
+  std::map<std::pair<int, int>, int> tile_index;
+  for (const auto& kv : pairs) {
+    const double sv = kv.second.first, se = kv.second.second;
+    if (sv == 1.0 && se == 1.0) continue;
+    int cls = -1;
+    for (size_t c = 0; c < plan.classes.size(); ++c)
+      if ((float)plan.classes[c].first == (float)sv && (float)plan.classes[c].second == (float)se) cls = (int)c;
+    if (cls < 0) {
+      if ((int)plan.classes.size() >= kMaxExclClasses)
+        throw std::invalid_argument("more than 3 distinct non-unit exclusion scale pairs in one system");
+      plan.classes.push_back({sv, se});
+      cls = (int)plan.classes.size() - 1;
+    }
+    const int code = cls + 1;  // 1..3
+    const int i = kv.first.first, j = kv.first.second;
+    const int I = i / kTile, J = j / kTile;
+    auto it = tile_index.find({I, J});
+    int t;
+    if (it == tile_index.end()) {
+      t = (int)plan.tiles.size();
+      tile_index[{I, J}] = t;
+      plan.tiles.push_back({I, J});
+      plan.tile_masks.resize(plan.tile_masks.size() + 128, 0ull);
+    } else {
+      t = it->second;
+    }
+    unsigned long long* m = &plan.tile_masks[(size_t)t * 128];
+    auto setbit = [&](int lane, int bit) {
+      if (code & 1) m[lane * 2 + 0] |= (1ull << bit);
+      if (code & 2) m[lane * 2 + 1] |= (1ull << bit);
+    };
+    setbit(i - I * kTile, j - J * kTile);
+    if (I == J) setbit(j - J * kTile, i - I * kTile);
+  }
+}
+
+void build_items(int n_atoms, int a0, int slot, const ExclPlan& plan, int excl_base,
+                 std::vector<PairItem>& out) {
+  const int nT = (n_atoms + kTile - 1) / kTile;
+  std::map<std::pair<int, int>, int> tix;
+  for (size_t t = 0; t < plan.tiles.size(); ++t) tix[plan.tiles[t]] = (int)t;
+  for (int I = 0; I < nT; ++I)
+    for (int J = I; J < nT; ++J) {
+      const int ni = std::min(kTile, n_atoms - I * kTile);
+      const int nj = std::min(kTile, n_atoms - J * kTile);
+      int flags = (I == J) ? kFlagDiag : 0;
+      int ex = -1;
+      auto it = tix.find({I, J});
+      if (it != tix.end()) {
+        flags |= kFlagExcl;
+        ex = excl_base + it->second;
+      }
+      PairItem p;
+      p.i0 = a0 + I * kTile;
+      p.j0 = a0 + J * kTile;
+      p.meta = item_meta(ni, nj, flags, slot);
+      p.excl = ex;
+      out.push_back(p);
+    }
+}
+
+}  // namespace prx

@@ -1,0 +1,410 @@
+// All-pairs passes of the GB-OBC2 energy/force evaluation on gfx950.
+//
+// One wave64 per work item (a 64x64 tile pair I <= J of one system). Lane l
+// owns i atom I0 + l; the 64 j atoms of tile J live one per lane in registers
+// and rotate one lane per iteration with a DPP wave rotate, together with
+// their force accumulators, so every lane meets every j exactly once and no
+// two lanes touch the same j in one iteration. After the loop each lane adds
+// its i and j partial sums to global 64-bit fixed-point accumulators: integer
+// adds commute, so results are bitwise reproducible in any launch order.
+//
+// Pass 1 (Born sums, generalized_born.py:211-263 with :32-94):
+//     I_i += I(r; rho_i, sr_j),  I_j += I(r; rho_j, sr_i)
+// Pass 2 (GB + Coulomb + LJ forces and dE/dB, generalized_born.py:97-208,
+//     optimization.py:21-297): GB over every pair, Coulomb/LJ with exclusion
+//     scales and the optional cutoff.
+// Pass 3 (Born chain rule, generalized_born.py:651-706):
+//     F -= (c_i dI_ij/dr + c_j dI_ji/dr) dr/dx, c = dE/dB * dB/dI.
+//
+// Diagonal tiles (I == J) visit each unordered pair once: offsets k = 1..31
+// on every lane and k = 32 on lanes 0..31 only.
+#include "prx_common.h"
+
+namespace prx {
+
+// DPP wave_rol:1 (lane l reads lane l + 1 mod 64). After k rotations lane l
+// holds the value that started in lane (l + k) & 63.
+#define PRX_DPP_WAVE_ROL1 0x134
+
+__device__ __forceinline__ float rot(float x) {
+  // Every lane has a valid source under a full-wave rotate, so no "old" value is needed.
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), PRX_DPP_WAVE_ROL1, 0xF, 0xF, true));
+}
+
+__device__ __forceinline__ void flush_fixed(unsigned long long* p, float x, double scale) {
+  long long q = __double2ll_rn((double)x * scale);
+  if (q != 0) atomicAdd(p, (unsigned long long)q);
+}
+
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float flog2(float x) { return __builtin_amdgcn_logf(x); }
+__device__ __forceinline__ float fexp2(float x) { return __builtin_amdgcn_exp2f(x); }
+
+__device__ __forceinline__ int wave_id() {
+  return __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)));
+}
+
+// ---------------------------------------------------------------------------
+// OBC2 pair integral pieces
+// ---------------------------------------------------------------------------
+
+// Both directions of one pair share r and one reciprocal.
+struct BornPair {
+  float I1, I2;  // I(r; rho_i, s_j) and I(r; rho_j, s_i)
+};
+
+__device__ __forceinline__ BornPair born_pair(float r, float rinv, float rho_i, float s_i, float s_i2,
+                                              float rho_j, float s_j, float s_j2) {
+  const float U1 = r + s_j, U2 = r + s_i;
+  const float L1 = fmaxf(fmaxf(rho_i, fabsf(r - s_j)), 1e-4f);
+  const float L2 = fmaxf(fmaxf(rho_j, fabsf(r - s_i)), 1e-4f);
+  const float U1c = fmaxf(U1, 1e-4f), U2c = fmaxf(U2, 1e-4f);
+  const float p1 = L1 * U1c, p2 = L2 * U2c;
+  const float inv = frcp(p1 * p2);
+  const float ip1 = inv * p2, ip2 = inv * p1;
+  const float iL1 = U1c * ip1, iU1 = L1 * ip1;
+  const float iL2 = U2c * ip2, iU2 = L2 * ip2;
+  const float rs = fminf(rinv, 1e4f);  // 1 / max(r, 1e-4)
+  const float lg1 = flog2(L1 * iU1), lg2 = flog2(L2 * iU2);
+  const float a1 = iL1 - iU1, a2 = iL2 - iU2;
+  // 1/U^2 - 1/L^2 = -(1/L - 1/U)(1/L + 1/U)
+  const float b1 = -a1 * (iL1 + iU1), b2 = -a2 * (iL2 + iU2);
+  const float c1 = fmaf(-s_j2, rs, r), c2 = fmaf(-s_i2, rs, r);
+  float I1 = fmaf(0.5f, a1, fmaf(0.25f * kLn2 * lg1, rs, 0.125f * c1 * b1));
+  float I2 = fmaf(0.5f, a2, fmaf(0.25f * kLn2 * lg2, rs, 0.125f * c2 * b2));
+  BornPair o;
+  o.I1 = (U1 > rho_i) ? I1 : 0.0f;
+  o.I2 = (U2 > rho_j) ? I2 : 0.0f;
+  return o;
+}
+
+// dI/dr for one direction (rho = rho_i, s = sr_j), the derivative of born_pair's I.
+__device__ __forceinline__ float born_dr(float r, float rs, float rho, float s, float s2) {
+  const float dd = r - s;
+  const float D = fabsf(dd);
+  const bool useD = D > rho;
+  const float L = useD ? D : rho;
+  const float Lc = fmaxf(L, 1e-4f);
+  float Lp = useD ? copysignf(1.0f, dd) : 0.0f;
+  Lp = (L > 1e-4f) ? Lp : 0.0f;
+  const float U = r + s;
+  const float Uc = fmaxf(U, 1e-4f);
+  const float Up = (U > 1e-4f) ? 1.0f : 0.0f;
+  const float rp = (rs < 1e4f) ? 1.0f : 0.0f;
+  const float inv = frcp(Lc * Uc);
+  const float iL = Uc * inv, iU = Lc * inv;
+  const float lg = flog2(Lc * iU) * kLn2;
+  const float iL2 = iL * iL, iU2 = iU * iU;
+  const float d1 = 0.5f * (Up * iU2 - Lp * iL2);
+  const float d2 = 0.25f * rs * ((Lp * iL - Up * iU) - lg * rp * rs);
+  const float d3 = 0.125f * fmaf(s2 * rp, rs * rs, 1.0f) * (iU2 - iL2);
+  const float d4 = 0.25f * fmaf(-s2, rs, r) * (Lp * iL2 * iL - Up * iU2 * iU);
+  // Every intermediate is finite (Lc, Uc >= 1e-4), so a 0/1 factor is exact
+  // and keeps the wave free of a divergent branch.
+  const float on = (U > rho) ? 1.0f : 0.0f;
+  return on * (d1 + d2 + d3 + d4);
+}
+
+// ---------------------------------------------------------------------------
+// Pass 1: Born sums
+// ---------------------------------------------------------------------------
+
+template <bool GENERAL, bool DIAG>
+__device__ __forceinline__ void born_tile(const DeviceView& v, const PairItem& it, int lane) {
+  const int ni = item_ni(it.meta), nj = item_nj(it.meta);
+  const int gi = it.i0 + min(lane, ni - 1);
+  const int gj = it.j0 + min(lane, nj - 1);
+  const float4 pi = v.pos[gi], bi = v.born4[gi];
+  float4 pj = v.pos[gj], bj = v.born4[gj];
+  float xj = pj.x, yj = pj.y, zj = pj.z, rhoj = bj.x, sj = bj.y, sj2 = bj.z;
+  float accI = 0.0f, accJ = 0.0f;
+  const bool ivalid = lane < ni;
+  int kbeg = 0, kend = 63;
+  if (DIAG) {
+    xj = rot(xj); yj = rot(yj); zj = rot(zj); rhoj = rot(rhoj); sj = rot(sj); sj2 = rot(sj2);
+    kbeg = 1; kend = 32;
+  }
+  for (int k = kbeg; k <= kend; ++k) {
+    const float dx = xj - pi.x, dy = yj - pi.y, dz = zj - pi.z;
+    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+    const float rinv = frsq(r2);
+    const float r = r2 * rinv;
+    BornPair bp = born_pair(r, rinv, bi.x, bi.y, bi.z, rhoj, sj, sj2);
+    if (GENERAL) {
+      const int orig = (lane + k) & 63;
+      bool valid = ivalid && orig < nj;
+      if (DIAG && k == 32) valid = valid && lane < 32;
+      bp.I1 = valid ? bp.I1 : 0.0f;
+      bp.I2 = valid ? bp.I2 : 0.0f;
+    }
+    accI += bp.I1;
+    accJ += bp.I2;
+    if (k != kend) {
+      xj = rot(xj); yj = rot(yj); zj = rot(zj); rhoj = rot(rhoj); sj = rot(sj); sj2 = rot(sj2);
+      accJ = rot(accJ);
+    }
+  }
+  const int nrot = DIAG ? 32 : 63;
+  const int home = (lane + nrot) & 63;
+  if (ivalid) flush_fixed(&v.iacc[it.i0 + lane], accI, kBornScale);
+  if (home < nj) flush_fixed(&v.iacc[it.j0 + home], accJ, kBornScale);
+}
+
+__global__ __launch_bounds__(256) void k_born_pairs(DeviceView v) {
+  const int w = wave_id();
+  if (w >= v.n_items) return;
+  const PairItem it = v.items[w];
+  const int lane = threadIdx.x & 63;
+  const int fl = item_flags(it.meta);
+  const bool full = item_ni(it.meta) == kTile && item_nj(it.meta) == kTile;
+  if (fl & kFlagDiag) {
+    born_tile<true, true>(v, it, lane);
+  } else if (full) {
+    born_tile<false, false>(v, it, lane);
+  } else {
+    born_tile<true, false>(v, it, lane);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Pass 2: GB + Coulomb + LJ
+// ---------------------------------------------------------------------------
+
+template <bool GENERAL, bool DIAG, bool ENERGY>
+__device__ __forceinline__ void force_tile(const DeviceView& v, const PairItem& it, int lane, int w) {
+  const int ni = item_ni(it.meta), nj = item_nj(it.meta);
+  const SlotConst& S = v.slots[item_slot(it.meta)];
+  const float kc = S.k_coulomb, gb2 = S.gb2, cut2 = S.cutoff2;
+  const bool has_cut = GENERAL && cut2 > 0.0f;
+  const bool has_excl = GENERAL && it.excl >= 0;
+  const int gi = it.i0 + min(lane, ni - 1);
+  const int gj = it.j0 + min(lane, nj - 1);
+  const float4 pi = v.pos[gi], nbi = v.nb4[gi], gbi = v.gb4[gi];
+  const float4 pj = v.pos[gj], nbj = v.nb4[gj], gbj = v.gb4[gj];
+  float xj = pj.x, yj = pj.y, zj = pj.z, qj = nbj.x, hsj = nbj.y, sej = nbj.z, Bj = gbj.x, aj = gbj.y;
+  float fxi = 0.f, fyi = 0.f, fzi = 0.f, dbi = 0.f;
+  float fxj = 0.f, fyj = 0.f, fzj = 0.f, dbj = 0.f;
+  double egb = 0.0, ecl = 0.0, elj = 0.0;
+  unsigned long long m0 = 0, m1 = 0;
+  if (has_excl) {
+    m0 = v.excl_masks[(size_t)it.excl * 128 + lane * 2];
+    m1 = v.excl_masks[(size_t)it.excl * 128 + lane * 2 + 1];
+  }
+  const bool ivalid = lane < ni;
+  int kbeg = 0, kend = 63;
+  if (DIAG) {
+    xj = rot(xj); yj = rot(yj); zj = rot(zj); qj = rot(qj); hsj = rot(hsj); sej = rot(sej);
+    Bj = rot(Bj); aj = rot(aj);
+    kbeg = 1; kend = 32;
+  }
+  for (int k = kbeg; k <= kend; ++k) {
+    const float dx = xj - pi.x, dy = yj - pi.y, dz = zj - pi.z;
+    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+    const float rinv = frsq(r2);
+    const float rinv2 = rinv * rinv;
+    const float qq = nbi.x * qj;
+    // Coulomb + LJ
+    float ec = kc * qq * rinv;
+    const float sg = nbi.y + hsj;
+    const float ep = nbi.z * sej;
+    const float s2 = sg * sg * rinv2;
+    const float s6 = s2 * s2 * s2;
+    float el = 4.0f * ep * (s6 * s6 - s6);
+    float fl = ep * s6 * fmaf(48.0f, s6, -24.0f);
+    // GB over every pair
+    const float BB = gbi.x * Bj;
+    const float x = r2 * (gbi.y * aj);  // r^2 log2(e) / (4 B_i B_j)
+    const float ex = fexp2(-x);
+    const float f2 = fmaf(BB, ex, r2);
+    const float finv = frsq(f2);
+    const float eg = gb2 * qq * finv;
+    const float t = eg * finv * finv;                 // gb2 qq / f^3
+    float fgb = t * fmaf(-0.25f, ex, 1.0f);            // -dE/dr / r
+    float g = -0.5f * t * ex * fmaf(x, kLn2, 1.0f);    // dE/dB_i = g B_j, dE/dB_j = g B_i
+    if (GENERAL) {
+      const int orig = (lane + k) & 63;
+      bool valid = ivalid && orig < nj;
+      if (DIAG && k == 32) valid = valid && lane < 32;
+      float sv = 1.0f, se = 1.0f;
+      if (has_excl) {
+        const int cls = (int)((m0 >> orig) & 1ull) | ((int)((m1 >> orig) & 1ull) << 1);
+        sv = cls == 0 ? 1.0f : (cls == 1 ? S.excl_v[1] : (cls == 2 ? S.excl_v[2] : S.excl_v[3]));
+        se = cls == 0 ? 1.0f : (cls == 1 ? S.excl_e[1] : (cls == 2 ? S.excl_e[2] : S.excl_e[3]));
+      }
+      bool clv = valid && (sv != 0.0f || se != 0.0f);
+      if (has_cut) clv = clv && r2 < cut2;
+      ec = clv ? ec * se : 0.0f;
+      el = clv ? el * sv : 0.0f;
+      fl = clv ? fl * sv : 0.0f;
+      fgb = valid ? fgb : 0.0f;
+      g = valid ? g : 0.0f;
+      if (ENERGY) egb += valid ? (double)eg : 0.0;
+    } else {
+      if (ENERGY) egb += (double)eg;
+    }
+    if (ENERGY) { ecl += (double)ec; elj += (double)el; }
+    float fr = fmaf(ec + fl, rinv2, fgb);
+    if (GENERAL) {
+      // An idle lane of a ragged tile may pair an atom with itself (r = 0):
+      // 0 * inf there is NaN, so the final scalar is selected, not just its parts.
+      const int orig = (lane + k) & 63;
+      bool valid = ivalid && orig < nj;
+      if (DIAG && k == 32) valid = valid && lane < 32;
+      fr = valid ? fr : 0.0f;
+    }
+    // F_i = -fr * (x_j - x_i)
+    fxi = fmaf(-fr, dx, fxi); fyi = fmaf(-fr, dy, fyi); fzi = fmaf(-fr, dz, fzi);
+    fxj = fmaf(fr, dx, fxj); fyj = fmaf(fr, dy, fyj); fzj = fmaf(fr, dz, fzj);
+    dbi = fmaf(g, Bj, dbi);
+    dbj = fmaf(g, gbi.x, dbj);
+    if (k != kend) {
+      xj = rot(xj); yj = rot(yj); zj = rot(zj); qj = rot(qj); hsj = rot(hsj); sej = rot(sej);
+      Bj = rot(Bj); aj = rot(aj);
+      fxj = rot(fxj); fyj = rot(fyj); fzj = rot(fzj); dbj = rot(dbj);
+    }
+  }
+  const int nrot = DIAG ? 32 : 63;
+  const int home = (lane + nrot) & 63;
+  if (ivalid) {
+    unsigned long long* a = &v.acc[(size_t)(it.i0 + lane) * 4];
+    flush_fixed(a + 0, fxi, kForceScale);
+    flush_fixed(a + 1, fyi, kForceScale);
+    flush_fixed(a + 2, fzi, kForceScale);
+    flush_fixed(a + 3, dbi, kForceScale);
+  }
+  if (home < nj) {
+    unsigned long long* a = &v.acc[(size_t)(it.j0 + home) * 4];
+    flush_fixed(a + 0, fxj, kForceScale);
+    flush_fixed(a + 1, fyj, kForceScale);
+    flush_fixed(a + 2, fzj, kForceScale);
+    flush_fixed(a + 3, dbj, kForceScale);
+  }
+  if (ENERGY) {
+    for (int off = 32; off > 0; off >>= 1) {
+      egb += __shfl_xor(egb, off);
+      ecl += __shfl_xor(ecl, off);
+      elj += __shfl_xor(elj, off);
+    }
+    if (lane == 0) {
+      v.e_items[(size_t)w * 3 + 0] = egb;
+      v.e_items[(size_t)w * 3 + 1] = ecl;
+      v.e_items[(size_t)w * 3 + 2] = elj;
+    }
+  }
+}
+
+template <bool ENERGY>
+__global__ __launch_bounds__(256) void k_force_pairs(DeviceView v) {
+  const int w = wave_id();
+  if (w >= v.n_items) return;
+  const PairItem it = v.items[w];
+  const int lane = threadIdx.x & 63;
+  const int fl = item_flags(it.meta);
+  const bool full = item_ni(it.meta) == kTile && item_nj(it.meta) == kTile;
+  const bool plain = full && !(fl & kFlagExcl) && v.slots[item_slot(it.meta)].cutoff2 == 0.0f;
+  if (fl & kFlagDiag) {
+    force_tile<true, true, ENERGY>(v, it, lane, w);
+  } else if (plain) {
+    force_tile<false, false, ENERGY>(v, it, lane, w);
+  } else {
+    force_tile<true, false, ENERGY>(v, it, lane, w);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Pass 3: chain rule through the Born radii
+// ---------------------------------------------------------------------------
+
+template <bool GENERAL, bool DIAG>
+__device__ __forceinline__ void chain_tile(const DeviceView& v, const PairItem& it, int lane) {
+  const int ni = item_ni(it.meta), nj = item_nj(it.meta);
+  const int gi = it.i0 + min(lane, ni - 1);
+  const int gj = it.j0 + min(lane, nj - 1);
+  const float4 pi = v.pos[gi], bi = v.born4[gi];
+  const float ci = v.gb4[gi].w;
+  const float4 pj = v.pos[gj], bj = v.born4[gj];
+  float xj = pj.x, yj = pj.y, zj = pj.z, rhoj = bj.x, sj = bj.y, sj2 = bj.z, cj = v.gb4[gj].w;
+  float fxi = 0.f, fyi = 0.f, fzi = 0.f, fxj = 0.f, fyj = 0.f, fzj = 0.f;
+  const bool ivalid = lane < ni;
+  int kbeg = 0, kend = 63;
+  if (DIAG) {
+    xj = rot(xj); yj = rot(yj); zj = rot(zj); rhoj = rot(rhoj); sj = rot(sj); sj2 = rot(sj2); cj = rot(cj);
+    kbeg = 1; kend = 32;
+  }
+  for (int k = kbeg; k <= kend; ++k) {
+    const float dx = xj - pi.x, dy = yj - pi.y, dz = zj - pi.z;
+    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+    const float rinv = frsq(r2);
+    const float r = r2 * rinv;
+    const float rs = fminf(rinv, 1e4f);
+    const float d1 = born_dr(r, rs, bi.x, sj, sj2);     // dI(r; rho_i, s_j)/dr
+    const float d2 = born_dr(r, rs, rhoj, bi.y, bi.z);  // dI(r; rho_j, s_i)/dr
+    float gsc = fmaf(ci, d1, cj * d2) * rinv;           // F_i = gsc * (x_j - x_i)
+    if (GENERAL) {
+      const int orig = (lane + k) & 63;
+      bool valid = ivalid && orig < nj;
+      if (DIAG && k == 32) valid = valid && lane < 32;
+      gsc = valid ? gsc : 0.0f;
+    }
+    fxi = fmaf(gsc, dx, fxi); fyi = fmaf(gsc, dy, fyi); fzi = fmaf(gsc, dz, fzi);
+    fxj = fmaf(-gsc, dx, fxj); fyj = fmaf(-gsc, dy, fyj); fzj = fmaf(-gsc, dz, fzj);
+    if (k != kend) {
+      xj = rot(xj); yj = rot(yj); zj = rot(zj); rhoj = rot(rhoj); sj = rot(sj); sj2 = rot(sj2);
+      cj = rot(cj);
+      fxj = rot(fxj); fyj = rot(fyj); fzj = rot(fzj);
+    }
+  }
+  const int nrot = DIAG ? 32 : 63;
+  const int home = (lane + nrot) & 63;
+  if (ivalid) {
+    unsigned long long* a = &v.acc[(size_t)(it.i0 + lane) * 4];
+    flush_fixed(a + 0, fxi, kForceScale);
+    flush_fixed(a + 1, fyi, kForceScale);
+    flush_fixed(a + 2, fzi, kForceScale);
+  }
+  if (home < nj) {
+    unsigned long long* a = &v.acc[(size_t)(it.j0 + home) * 4];
+    flush_fixed(a + 0, fxj, kForceScale);
+    flush_fixed(a + 1, fyj, kForceScale);
+    flush_fixed(a + 2, fzj, kForceScale);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_chain_pairs(DeviceView v) {
+  const int w = wave_id();
+  if (w >= v.n_items) return;
+  const PairItem it = v.items[w];
+  const int lane = threadIdx.x & 63;
+  const int fl = item_flags(it.meta);
+  const bool full = item_ni(it.meta) == kTile && item_nj(it.meta) == kTile;
+  if (fl & kFlagDiag) {
+    chain_tile<true, true>(v, it, lane);
+  } else if (full) {
+    chain_tile<false, false>(v, it, lane);
+  } else {
+    chain_tile<true, false>(v, it, lane);
+  }
+}
+
+static inline int pair_blocks(int n_items) { return (n_items + 3) / 4; }
+
+void launch_born_pairs(const DeviceView& v, hipStream_t s) {
+  if (v.n_items == 0) return;
+  hipLaunchKernelGGL(k_born_pairs, dim3(pair_blocks(v.n_items)), dim3(256), 0, s, v);
+}
+
+void launch_force_pairs(const DeviceView& v, bool energy, hipStream_t s) {
+  if (v.n_items == 0) return;
+  if (energy)
+    hipLaunchKernelGGL(k_force_pairs<true>, dim3(pair_blocks(v.n_items)), dim3(256), 0, s, v);
+  else
+    hipLaunchKernelGGL(k_force_pairs<false>, dim3(pair_blocks(v.n_items)), dim3(256), 0, s, v);
+}
+
+void launch_chain_pairs(const DeviceView& v, hipStream_t s) {
+  if (v.n_items == 0) return;
+  hipLaunchKernelGGL(k_chain_pairs, dim3(pair_blocks(v.n_items)), dim3(256), 0, s, v);
+}
+
+}  // namespace prx

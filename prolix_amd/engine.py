@@ -1,0 +1,151 @@
+"""Python handle on the native engine: packed systems, energies/forces, BAOAB.
+
+Device memory and streams come from PyTorch (ROCm build): every per-step
+array is a ``torch.float32`` CUDA tensor whose pointer is handed to the C ABI,
+and work is ordered on the current torch stream. Nothing here computes
+physics; it only moves buffers and calls the HIP kernels.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from prolix_amd import _native as N
+
+
+def _torch():
+    import torch
+
+    if not torch.cuda.is_available():
+        raise RuntimeError("prolix_amd's engine needs a ROCm GPU (torch.cuda.is_available() is False)")
+    return torch
+
+
+def _stream(torch):
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _vp(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+@dataclass
+class Langevin:
+    """BAOAB parameters of one system (fs, K, 1/ps, Philox seed)."""
+
+    dt_fs: float = 1.0
+    temperature: float = 300.0
+    gamma_ps: float = 1.0
+    seed: int = 0
+
+    def c(self) -> N.LangevinParams:
+        return N.LangevinParams(float(self.dt_fs), float(self.temperature), float(self.gamma_ps),
+                                int(self.seed) & 0xFFFFFFFFFFFFFFFF)
+
+
+class GBEngine:
+    """Many systems packed into one engine (a single system is a pack of one)."""
+
+    def __init__(self, systems, *, cutoff: float = 0.0, coulomb_constant: float = 332.0637,
+                 gb_coulomb_constant: float = 332.0637, solute_dielectric: float = 1.0,
+                 solvent_dielectric: float = 78.5, dielectric_offset: float = 0.09,
+                 implicit_solvent: bool = True):
+        torch = _torch()
+        self.torch = torch
+        if isinstance(systems, dict):
+            systems = [systems]
+        opts = dict(cutoff=cutoff, coulomb_constant=coulomb_constant, gb_coulomb_constant=gb_coulomb_constant,
+                    solute_dielectric=solute_dielectric, solvent_dielectric=solvent_dielectric,
+                    dielectric_offset=dielectric_offset, implicit_solvent=implicit_solvent)
+        self._descs = [N.Desc(s, **opts) for s in systems]
+        arr = (N.SystemDesc * len(self._descs))(*[d.desc for d in self._descs])
+        h = ctypes.c_void_p()
+        N.check(N.lib().prx_system_create(arr, len(self._descs), ctypes.byref(h)), "prx_system_create")
+        self._h = h
+        self.n_systems = len(self._descs)
+        off = (ctypes.c_int32 * (self.n_systems + 1))()
+        N.check(N.lib().prx_engine_layout(h, off, self.n_systems + 1), "prx_engine_layout")
+        self.offsets = np.array(off[:], dtype=np.int64)
+        self.n_atoms = int(self.offsets[-1])
+        self.sizes = np.diff(self.offsets)
+        self.device = torch.device("cuda")
+
+    # ---- helpers --------------------------------------------------------
+    def pack(self, positions_list):
+        """Concatenate per-system (n_s, 3) arrays into one device tensor."""
+        t = self.torch
+        parts = [t.as_tensor(np.asarray(p, np.float32).reshape(-1, 3)) for p in positions_list]
+        return t.cat(parts).to(self.device).contiguous()
+
+    def unpack(self, packed):
+        a = packed.detach().cpu().numpy() if hasattr(packed, "detach") else np.asarray(packed)
+        return [a[self.offsets[s]:self.offsets[s + 1]] for s in range(self.n_systems)]
+
+    def _pos_tensor(self, positions):
+        t = self.torch
+        if isinstance(positions, (list, tuple)):
+            return self.pack(positions)
+        x = t.as_tensor(positions, dtype=t.float32, device=self.device).reshape(-1, 3).contiguous()
+        if x.shape[0] != self.n_atoms:
+            raise ValueError(f"positions have {x.shape[0]} atoms, engine has {self.n_atoms}")
+        return x
+
+    # ---- energy / forces ------------------------------------------------
+    def energy_forces(self, positions, *, energies: bool = True, born_radii: bool = False):
+        """Returns (energies (n_systems, 12) float64 | None, forces (N, 3) float32, born | None)."""
+        t = self.torch
+        x = self._pos_tensor(positions)
+        f = t.empty((self.n_atoms, 3), dtype=t.float32, device=self.device)
+        e = t.empty((self.n_systems, N.N_ENERGY), dtype=t.float64, device=self.device) if energies else None
+        b = t.empty((self.n_atoms,), dtype=t.float32, device=self.device) if born_radii else None
+        N.check(N.lib().prx_energy_forces(self._h, _vp(x), _vp(f), _vp(e), _vp(b), _stream(t)), "prx_energy_forces")
+        return e, f, b
+
+    # ---- integrator -----------------------------------------------------
+    def init(self, positions, params, momenta=None):
+        t = self.torch
+        if isinstance(params, Langevin):
+            params = [params] * self.n_systems
+        if len(params) != self.n_systems:
+            raise ValueError("one Langevin parameter set per system is required")
+        x = self._pos_tensor(positions)
+        m = None if momenta is None else self._pos_tensor(momenta)
+        arr = (N.LangevinParams * self.n_systems)(*[p.c() for p in params])
+        N.check(N.lib().prx_integrator_init(self._h, _vp(x), _vp(m), arr, _stream(t)), "prx_integrator_init")
+
+    def steps(self, n_steps: int):
+        N.check(N.lib().prx_integrator_steps(self._h, int(n_steps), _stream(self.torch)), "prx_integrator_steps")
+
+    def state(self):
+        t = self.torch
+        x = t.empty((self.n_atoms, 3), dtype=t.float32, device=self.device)
+        p = t.empty_like(x)
+        f = t.empty_like(x)
+        N.check(N.lib().prx_integrator_state(self._h, _vp(x), _vp(p), _vp(f), _stream(t)), "prx_integrator_state")
+        return x, p, f
+
+    def kinetic(self):
+        t = self.torch
+        ke = t.empty((self.n_systems,), dtype=t.float64, device=self.device)
+        N.check(N.lib().prx_integrator_kinetic(self._h, _vp(ke), _stream(t)), "prx_integrator_kinetic")
+        return ke
+
+    def step_count(self, system: int = 0) -> int:
+        c = ctypes.c_int64()
+        N.check(N.lib().prx_integrator_step_count(self._h, int(system), ctypes.byref(c)), "prx_integrator_step_count")
+        return c.value
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self.torch.cuda.synchronize()
+            N.check(N.lib().prx_engine_destroy(self._h), "prx_engine_destroy")
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,147 @@
+"""BAOAB on the device: zero-friction agreement with a float64 velocity-Verlet
+oracle, NVE drift, Langevin temperature, Philox stream invariances."""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle import gb_oracle as O
+from prolix_amd import synth
+from tests.helpers import f32_round, protein
+
+pytestmark = pytest.mark.gpu
+
+AKMA_FS = 48.88821291839
+KB = 0.0019872041
+
+
+def _engine(systems, **kw):
+    from prolix_amd.engine import GBEngine
+
+    return GBEngine(systems, **kw)
+
+
+def _relaxed(sysd, steps=100):
+    from prolix_amd.minimize import minimize
+
+    eng = _engine(sysd)
+    x = minimize(eng, sysd["positions"], steps=steps)
+    out = dict(sysd)
+    out["positions"] = x.cpu().numpy().astype(np.float64)
+    eng.close()
+    return out
+
+
+def test_zero_friction_matches_float64_verlet():
+    """gamma = 0 reduces BAOAB to velocity Verlet (settle.py:2064-2087)."""
+    from prolix_amd.engine import Langevin
+
+    print("[gpu] zero-friction trajectory, 138 atoms", flush=True)
+    sysd = _relaxed(protein(138, 0), steps=50)
+    n = len(sysd["charges"])
+    m = np.asarray(sysd["masses"], np.float64)
+    rng = np.random.default_rng(1)
+    p0 = (rng.normal(size=(n, 3)) * np.sqrt(m[:, None] * KB * 300.0)).astype(np.float32).astype(np.float64)
+    x0 = sysd["positions"]
+    steps, dt_fs = 50, 1.0
+    eng = _engine(sysd)
+    eng.init(x0, Langevin(dt_fs=dt_fs, temperature=300.0, gamma_ps=0.0, seed=5), momenta=p0)
+    eng.steps(steps)
+    xg, pg, fg = (a.cpu().numpy().astype(np.float64) for a in eng.state())
+    eng.close()
+    dt = dt_fs / AKMA_FS
+    ff = lambda x: O.forces(sysd, x)
+    x, p, f = x0.copy(), p0.copy(), ff(x0)
+    for _ in range(steps):
+        x, p, f = O.baoab_step(x, p, f, m, dt, 0.0, 0.0, np.zeros_like(x), ff)
+    dx = np.sqrt(np.mean((xg - x) ** 2))
+    dp = np.sqrt(np.mean((pg - p) ** 2) / np.mean(p**2))
+    print(f"[zero-friction] rms dx {dx:.2e} A, rel dp {dp:.2e}", flush=True)
+    assert dx < 1e-4
+    assert dp < 1e-3
+
+
+def test_nve_drift_three_particles():
+    """The reference's GB NVE check (test_generalized_born.py:138-275): three ions,
+    dt = 1e-3 time units, 100 steps, relative energy drift < 1e-4."""
+    from prolix_amd.engine import Langevin
+
+    sysd = {"positions": np.array([[0.0, 0.0, 0.0], [3.0, 0.0, 0.0], [0.0, 4.0, 0.0]]),
+            "charges": np.array([0.5, -0.5, 0.5]), "sigmas": np.array([1.0, 1.0, 1.0]),
+            "epsilons": np.zeros(3), "radii": np.array([1.5, 1.5, 1.5]), "masses": np.full(3, 10.0)}
+    sysd = f32_round(sysd)
+    rng = np.random.default_rng(0)
+    v0 = rng.normal(size=(3, 3)) * 0.1
+    p0 = v0 * 10.0
+    eng = _engine(sysd)
+    e0, _, _ = eng.energy_forces(sysd["positions"])
+    E0 = float(e0[0, -1]) + float(np.sum(p0**2 / 20.0))
+    eng.init(sysd["positions"], Langevin(dt_fs=1e-3 * AKMA_FS, temperature=0.0, gamma_ps=0.0), momenta=p0)
+    eng.steps(100)
+    x, p, _ = eng.state()
+    ke = float(eng.kinetic()[0])
+    e1, _, _ = eng.energy_forces(x)
+    E1 = float(e1[0, -1]) + ke
+    eng.close()
+    drift = abs(E1 - E0) / abs(E0)
+    print(f"[nve-3] drift {drift:.2e}", flush=True)
+    assert drift < 1e-4
+
+
+def test_chunking_and_batch_bitwise():
+    """Step results do not depend on how steps are split into graph launches,
+    nor on which other systems share the engine."""
+    from prolix_amd.engine import Langevin
+
+    systems = [protein(n, 40 + k, radius=synth.compact_radius(n)) for k, n in enumerate((130, 260))]
+    systems = [_relaxed(s, steps=30) for s in systems]
+    params = [Langevin(1.0, 300.0, 1.0, seed=100 + k) for k in range(2)]
+    print("[gpu] batch trajectory 100 steps", flush=True)
+    eng = _engine(systems)
+    eng.init([s["positions"] for s in systems], params)
+    eng.steps(100)
+    xb = eng.unpack(eng.state()[0])
+    eng.close()
+    eng = _engine(systems)
+    eng.init([s["positions"] for s in systems], params)
+    eng.steps(30)
+    eng.steps(70)
+    xc = eng.unpack(eng.state()[0])
+    assert eng.step_count(0) == 100
+    eng.close()
+    for k, s in enumerate(systems):
+        e1 = _engine(s)
+        e1.init(s["positions"], params[k])
+        e1.steps(100)
+        xa = e1.state()[0].cpu().numpy()
+        e1.close()
+        assert np.array_equal(xa, xb[k]), k
+        assert np.array_equal(xa, xc[k]), k
+
+
+def test_langevin_temperature_and_initial_momenta():
+    from prolix_amd.engine import Langevin
+
+    sysd = _relaxed(protein(300, 7, radius=synth.compact_radius(300)), steps=100)
+    n = len(sysd["charges"])
+    T = 320.0
+    eng = _engine(sysd)
+    eng.init(sysd["positions"], Langevin(dt_fs=1.0, temperature=T, gamma_ps=20.0, seed=9))
+    _, p, _ = eng.state()
+    p = p.cpu().numpy().astype(np.float64)
+    assert np.allclose(p.sum(0), 0.0, atol=1e-3)  # centred (jax_md initialize_momenta)
+    temps = []
+    print("[gpu] langevin 3000 steps", flush=True)
+    for _ in range(60):
+        eng.steps(50)
+        temps.append(2.0 * float(eng.kinetic()[0]) / (3 * n * KB))
+    eng.close()
+    tail = np.array(temps[20:])
+    print(f"[langevin] T mean {tail.mean():.1f} sd {tail.std():.1f} (target {T})", flush=True)
+    assert np.all(np.isfinite(tail))
+    assert abs(tail.mean() - T) / T < 0.06
+    # per-sample spread of a 3N-dof kinetic temperature is ~ T sqrt(2 / 3N)
+    assert tail.std() < 3.0 * T * math.sqrt(2.0 / (3 * n))
