@@ -113,10 +113,14 @@ typedef struct prx_system_desc {
 /* Langevin BAOAB parameters for one system (jax_md nvt_langevin as used at
  * physics/simulate.py:245-255; AKMA conversion as simulate.py:770-772). */
 typedef struct prx_langevin_params {
-  double dt_fs;       /* time step, fs */
-  double temperature; /* K */
-  double gamma_ps;    /* friction, 1/ps (0 = velocity Verlet) */
-  uint64_t seed;      /* Philox4x32-10 key of this system's noise stream */
+  double dt_fs;        /* time step, fs */
+  double temperature;  /* K */
+  double gamma_ps;     /* friction, 1/ps (0 = velocity Verlet) */
+  uint64_t seed;       /* Philox4x32-10 key of this system's noise stream */
+  int32_t safety_caps; /* 1: the batched step's caps (batched_simulate.py:269-358):
+                          |p/m| <= 20 A/tau, |dx| <= 0.5 A per step, |F| <= 1e4
+                          kcal/mol/A, non-finite forces -> 0. 0: plain nvt_langevin */
+  int32_t reserved;
 } prx_langevin_params;
 
 typedef struct prx_engine prx_engine;
@@ -164,6 +168,22 @@ int prx_integrator_state(prx_engine* eng, float* positions, float* momenta, floa
 int prx_integrator_kinetic(prx_engine* eng, double* kinetic, void* stream);
 /* Steps taken so far by system s (host value, synchronises the stream). */
 int prx_integrator_step_count(prx_engine* eng, int32_t system, int64_t* steps);
+/* Safety-cap event counters of system s (atom-steps that hit the velocity,
+ * force, constraint and displacement caps; warn_counts of typing.py:558-563).
+ * Host out, 4 values; synchronises the stream. */
+int prx_integrator_warnings(prx_engine* eng, int32_t system, int64_t* counts);
+
+/* ---- measurement --------------------------------------------------------
+ * Launch each force pass of one step on its own, n_reps times, bracketed by
+ * HIP events on the engine stream, at the current positions; ms receives the
+ * mean duration per launch of: Born pairs, Born gather, force pairs, force
+ * gather, chain pairs, bonded, chain gather (7 doubles). The integrator's
+ * positions and momenta are untouched; its stored forces are recomputed at
+ * the same positions.
+ * No counterpart in prolix (its timing is wall-clock around jitted calls). */
+int prx_profile_passes(prx_engine* eng, int32_t n_reps, double* ms);
+/* Number of pair work items (64x64 tiles) of the engine's current work list. */
+int prx_engine_work_items(const prx_engine* eng, int64_t* n_items);
 
 /* ---- serving pool -------------------------------------------------------
  * A pool is an engine whose systems come and go. Capacities bound the

@@ -54,7 +54,7 @@ class SystemDesc(ctypes.Structure):
 
 class LangevinParams(ctypes.Structure):
     _fields_ = [("dt_fs", ctypes.c_double), ("temperature", ctypes.c_double), ("gamma_ps", ctypes.c_double),
-                ("seed", ctypes.c_uint64)]
+                ("seed", ctypes.c_uint64), ("safety_caps", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 class PoolLimits(ctypes.Structure):
@@ -77,6 +77,7 @@ _SIGS = {
                                             ctypes.c_void_p]),
     "prx_integrator_kinetic": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "prx_integrator_step_count": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64)]),
+    "prx_integrator_warnings": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64)]),
     "prx_pool_create": (ctypes.c_int, [ctypes.POINTER(PoolLimits), ctypes.POINTER(ctypes.c_void_p)]),
     "prx_pool_admit": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SystemDesc), c_f32p,
                                       ctypes.POINTER(LangevinParams), c_i32p, ctypes.c_void_p]),
@@ -85,6 +86,8 @@ _SIGS = {
                                      ctypes.POINTER(ctypes.c_int64), ctypes.c_void_p]),
     "prx_pool_slots": (ctypes.c_int, [ctypes.c_void_p, c_i32p, ctypes.c_int32, c_i32p]),
     "prx_pool_energy": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, c_f64p, ctypes.c_void_p]),
+    "prx_profile_passes": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, c_f64p]),
+    "prx_engine_work_items": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]),
     "prx_cmap_coefficients": (ctypes.c_int, [c_f64p, ctypes.c_int32, c_f64p]),
     "prx_plan_tiles": (ctypes.c_int, [ctypes.POINTER(SystemDesc), ctypes.POINTER(ctypes.c_int64),
                                       ctypes.POINTER(ctypes.c_int64)]),

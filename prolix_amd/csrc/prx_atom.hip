@@ -4,9 +4,6 @@
 
 namespace prx {
 
-__device__ __forceinline__ double fixed_to_double(unsigned long long v, double inv_scale) {
-  return (double)(long long)v * inv_scale;
-}
 
 // ---------------------------------------------------------------------------
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) and Box-Muller normals.
@@ -54,122 +51,204 @@ __device__ __forceinline__ float3 normal3(uint64_t seed, uint32_t step, int loca
 }
 
 // ---------------------------------------------------------------------------
+// Tile gathers: one wave per 64-atom tile T of a system with nT tiles. The
+// tile's atoms received i-side partials from the row items (T, J), J >= T,
+// and j-side partials from the column items (I, T), I <= T; items are stored
+// row-major per system, so item (I, J) = base + I nT - I (I - 1) / 2 + (J - I).
+// Sums run in that fixed order, in FP64.
+// ---------------------------------------------------------------------------
 
-__global__ void k_zero_acc(DeviceView v) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= v.atom_hw) return;
-  v.acc[(size_t)g * 4 + 0] = 0ull;
-  v.acc[(size_t)g * 4 + 1] = 0ull;
-  v.acc[(size_t)g * 4 + 2] = 0ull;
-  v.acc[(size_t)g * 4 + 3] = 0ull;
-  v.iacc[g] = 0ull;
+struct TileCtx {
+  int g;       // this lane's atom (arena index)
+  bool valid;  // lane < atoms in tile
+  int slot, T, nT, base, lane;
+};
+
+__device__ __forceinline__ int gather_wave() {
+  return __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)));
+}
+
+__device__ __forceinline__ TileCtx tile_ctx(const DeviceView& v, int t) {
+  const int4 tl = v.tiles[t];
+  TileCtx c;
+  c.lane = threadIdx.x & 63;
+  c.g = tl.x + c.lane;
+  c.valid = c.lane < tl.y;
+  c.slot = tl.z;
+  c.T = tl.w;
+  const SlotConst& S = v.slots[c.slot];
+  c.nT = (S.n + kTile - 1) / kTile;
+  c.base = S.item_base;
+  return c;
+}
+
+__device__ __forceinline__ int row_start(const TileCtx& c, int I) { return c.base + I * c.nT - (I * (I - 1)) / 2; }
+
+__device__ __forceinline__ size_t pslot(int w, int side, int lane) { return ((size_t)w * 2 + side) * kTile + lane; }
+
+// Scalar partials (Born sums) of this lane's atom.
+__device__ __forceinline__ double gather1(const DeviceView& v, const TileCtx& c) {
+  const float* pb = reinterpret_cast<const float*>(v.part);
+  double acc = 0.0;
+  const int r0 = row_start(c, c.T);
+  for (int J = c.T; J < c.nT; ++J) acc += (double)pb[pslot(r0 + (J - c.T), 0, c.lane)];
+  for (int I = 0; I <= c.T; ++I) acc += (double)pb[pslot(row_start(c, I) + (c.T - I), 1, c.lane)];
+  return acc;
+}
+
+__device__ __forceinline__ double4 gather4(const DeviceView& v, const TileCtx& c) {
+  double4 acc = make_double4(0.0, 0.0, 0.0, 0.0);
+  auto add = [&](float4 p) { acc.x += p.x; acc.y += p.y; acc.z += p.z; acc.w += p.w; };
+  const int r0 = row_start(c, c.T);
+  for (int J = c.T; J < c.nT; ++J) add(v.part[pslot(r0 + (J - c.T), 0, c.lane)]);
+  for (int I = 0; I <= c.T; ++I) add(v.part[pslot(row_start(c, I) + (c.T - I), 1, c.lane)]);
+  return acc;
 }
 
 // B_i from I_i in FP64 (generalized_born.py:81-93); dB/dI for the chain rule.
-__global__ void k_born_finish(DeviceView v) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= v.atom_hw || v.slot_of[g] < 0) return;
-  const double4 f = v.fin4[g];  // rho, R, q^2, ace K
-  const double I = fixed_to_double(v.iacc[g], 1.0 / kBornScale);
+__global__ __launch_bounds__(256) void k_born_gather(DeviceView v) {
+  const int t = gather_wave();
+  if (t >= v.n_tiles) return;
+  const TileCtx c = tile_ctx(v, t);
+  const double I = gather1(v, c);
+  if (!c.valid) return;
+  const double4 f = v.fin4[c.g];  // rho, R, q^2, ace K
   const double psi = f.x * I;
-  const double t = psi * (1.0 + psi * (-0.8 + 4.85 * psi));
-  const double th = tanh(t);
+  const double tt = psi * (1.0 + psi * (-0.8 + 4.85 * psi));
+  const double th = tanh(tt);
   const double inv_b = 1.0 / f.x - th / f.y;
   const double B = 1.0 / inv_b;
   const double dBdI = B * B * (1.0 - th * th) * (1.0 + psi * (-1.6 + 14.55 * psi)) * f.x / f.y;
-  float4 o;
-  o.x = (float)B;
-  o.y = (float)(0.6005612043932249 / B);  // sqrt(log2(e) / 4) / B
-  o.z = (float)dBdI;
-  o.w = 0.0f;
-  v.gb4[g] = o;
+  v.gb4[c.g] = make_float4((float)B, (float)(0.6005612043932249 / B), (float)dBdI, 0.0f);  // sqrt(log2(e)/4)/B
 }
 
-// c_i = dE/dB_i * dB_i/dI_i with the GB self term and ACE added (FP64).
+// Pair force and dE/dB; c_i = dE/dB_i * dB_i/dI_i with the GB self term and ACE (FP64).
 template <bool ENERGY>
-__global__ void k_chain_prep(DeviceView v) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= v.atom_hw) return;
-  const int s = v.slot_of[g];
-  if (s < 0) return;
-  const double4 f = v.fin4[g];
-  const float4 gb = v.gb4[g];
+__global__ __launch_bounds__(256) void k_force_gather(DeviceView v) {
+  const int t = gather_wave();
+  if (t >= v.n_tiles) return;
+  const TileCtx c = tile_ctx(v, t);
+  const double4 a = gather4(v, c);
+  if (!c.valid) return;
+  const double4 f = v.fin4[c.g];
+  const float4 gb = v.gb4[c.g];
   const double B = (double)gb.x;
-  const double pref = v.slots[s].gb_pref;
+  const double pref = v.slots[c.slot].gb_pref;
   const double e_self = pref * f.z / B;
   const double e_ace = f.w / (B * B * B * B * B * B);
-  const double dEdB = fixed_to_double(v.acc[(size_t)g * 4 + 3], 1.0 / kForceScale) - e_self / B
-                      - 6.0 * e_ace / B;
-  v.gb4[g].w = (float)(dEdB * (double)gb.z);
+  const double dEdB = a.w - e_self / B - 6.0 * e_ace / B;
+  v.gb4[c.g].w = (float)(dEdB * (double)gb.z);
+  v.fpair[c.g] = make_float4((float)a.x, (float)a.y, (float)a.z, 0.0f);
   if (ENERGY) {
-    v.e_atom[(size_t)g * 2 + 0] = e_self;
-    v.e_atom[(size_t)g * 2 + 1] = e_ace;
+    v.e_atom[(size_t)c.g * 2 + 0] = e_self;
+    v.e_atom[(size_t)c.g * 2 + 1] = e_ace;
   }
 }
 
-__device__ __forceinline__ float3 acc_force(const DeviceView& v, int g) {
-  const double inv = 1.0 / kForceScale;
-  float3 f;
-  f.x = (float)fixed_to_double(v.acc[(size_t)g * 4 + 0], inv);
-  f.y = (float)fixed_to_double(v.acc[(size_t)g * 4 + 1], inv);
-  f.z = (float)fixed_to_double(v.acc[(size_t)g * 4 + 2], inv);
-  return f;
+// Bonded records of atom g, summed in the fixed order of its record list.
+__device__ __forceinline__ float3 bonded_force(const DeviceView& v, int g) {
+  const int2 lr = v.atom_rec[g];
+  float bx = 0.0f, by = 0.0f, bz = 0.0f;
+  for (int k = 0; k < lr.y; ++k) {
+    const float4 r = v.rec[v.rec_list[lr.x + k]];
+    bx += r.x;
+    by += r.y;
+    bz += r.z;
+  }
+  return make_float3(bx, by, bz);
 }
 
-__global__ void k_gather_forces(DeviceView v) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= v.atom_hw || v.slot_of[g] < 0) return;
-  const float3 f = acc_force(v, g);
-  v.force[g] = make_float4(f.x, f.y, f.z, 0.0f);
+// BAOAB first half (settle.py:2064-2087): B(F_n) A O A for atom g of slot s.
+__device__ __forceinline__ void begin_step(const DeviceView& v, const SlotConst& S, int s, int g, float4& p,
+                                           uint32_t step) {
+  const float4 f = v.force[g];
+  p.x = fmaf(S.half_dt, f.x, p.x);
+  p.y = fmaf(S.half_dt, f.y, p.y);
+  p.z = fmaf(S.half_dt, f.z, p.z);
+  if (S.caps) {  // velocity limit (batched_simulate.py:273-278)
+    const float vm = sqrtf(p.x * p.x + p.y * p.y + p.z * p.z + 1e-30f) * p.w;
+    if (vm > kVLimit) {
+      const float sc = kVLimit / vm;
+      p.x *= sc; p.y *= sc; p.z *= sc;
+      atomicAdd(&v.warn[s * 4 + kWarnVLimit], 1u);
+    }
+  }
+  float4 x = v.pos[g];
+  const float4 x_old = x;
+  const float hv = S.half_dt * p.w;
+  x.x = fmaf(hv, p.x, x.x);
+  x.y = fmaf(hv, p.y, x.y);
+  x.z = fmaf(hv, p.z, x.z);
+  const float3 xi = normal3(S.seed, step, g - S.a0, 0u);
+  const float amp = S.c2 * S.sqrt_kT * v.sqrt_m[g];
+  p.x = fmaf(S.c1, p.x, amp * xi.x);
+  p.y = fmaf(S.c1, p.y, amp * xi.y);
+  p.z = fmaf(S.c1, p.z, amp * xi.z);
+  x.x = fmaf(hv, p.x, x.x);
+  x.y = fmaf(hv, p.y, x.y);
+  x.z = fmaf(hv, p.z, x.z);
+  if (S.caps) {  // displacement cap, momentum rescaled to match (batched_simulate.py:290-303)
+    const float dx = x.x - x_old.x, dy = x.y - x_old.y, dz = x.z - x_old.z;
+    const float dm = sqrtf(dx * dx + dy * dy + dz * dz + 1e-30f);
+    if (dm > kMaxDx) {
+      const float sc = kMaxDx / dm;
+      x.x = x_old.x + dx * sc; x.y = x_old.y + dy * sc; x.z = x_old.z + dz * sc;
+      p.x *= sc; p.y *= sc; p.z *= sc;
+      atomicAdd(&v.warn[s * 4 + kWarnDxCap], 1u);
+    }
+  }
+  v.pos[g] = x;
 }
 
-// BAOAB (settle.py:2064-2087, composed as settle.py:1699-1744):
-//   mode 0: B(F_n) A O A, zero the accumulators for the next force evaluation
-//   mode 2: gather F_{n+1}, B(F_{n+1}), count the step
-//   mode 1: mode 2 then mode 0 (the boundary between two steps)
-__global__ void k_kick_drift(DeviceView v, int mode) {
+// Total force = chain partials + pair force + bonded records; then
+//   mode -1: store it; 2: B(F) and count the step; 1: B(F), then B A O A.
+__global__ __launch_bounds__(256) void k_chain_gather(DeviceView v, int mode) {
+  const int t = gather_wave();
+  if (t >= v.n_tiles) return;
+  const TileCtx c = tile_ctx(v, t);
+  const double4 a = gather4(v, c);
+  if (!c.valid) return;
+  const float4 fp = v.fpair[c.g];
+  const float3 fb = bonded_force(v, c.g);
+  float3 f = make_float3((float)(a.x + (double)fp.x) + fb.x, (float)(a.y + (double)fp.y) + fb.y,
+                         (float)(a.z + (double)fp.z) + fb.z);
+  if (mode < 0) {
+    v.force[c.g] = make_float4(f.x, f.y, f.z, 0.0f);
+    return;
+  }
+  const SlotConst& S = v.slots[c.slot];
+  if (!S.active) return;
+  if (S.caps) {
+    // non-finite -> 0, then |F| <= kForceCap (batched_simulate.py:323-330)
+    if (!isfinite(f.x) || !isfinite(f.y) || !isfinite(f.z)) f = make_float3(0.f, 0.f, 0.f);
+    const float fm = sqrtf(f.x * f.x + f.y * f.y + f.z * f.z + 1e-12f);
+    if (fm > kForceCap) {
+      const float sc = kForceCap / fm;
+      f = make_float3(f.x * sc, f.y * sc, f.z * sc);
+      atomicAdd(&v.warn[c.slot * 4 + kWarnForceCap], 1u);
+    }
+  }
+  v.force[c.g] = make_float4(f.x, f.y, f.z, 0.0f);
+  float4 p = v.mom[c.g];  // w = 1/m
+  p.x = fmaf(S.half_dt, f.x, p.x);
+  p.y = fmaf(S.half_dt, f.y, p.y);
+  p.z = fmaf(S.half_dt, f.z, p.z);
+  const uint32_t step = v.atom_step[c.g] + 1u;  // each lane owns its atom's counter
+  v.atom_step[c.g] = step;
+  if (mode == 1) begin_step(v, S, c.slot, c.g, p, step);
+  v.mom[c.g] = p;
+}
+
+// B A O A from the stored force, for the first step of a launch.
+__global__ void k_step_begin(DeviceView v) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= v.atom_hw) return;
   const int s = v.slot_of[g];
   if (s < 0) return;
   const SlotConst& S = v.slots[s];
   if (!S.active) return;
-  float4 p = v.mom[g];  // w = 1/m
-  uint32_t step = v.atom_step[g];  // each thread owns its atom's counter
-  if (mode >= 1) {
-    const float3 f = acc_force(v, g);
-    v.force[g] = make_float4(f.x, f.y, f.z, 0.0f);
-    p.x = fmaf(S.half_dt, f.x, p.x);
-    p.y = fmaf(S.half_dt, f.y, p.y);
-    p.z = fmaf(S.half_dt, f.z, p.z);
-    step += 1u;
-    v.atom_step[g] = step;
-  }
-  if (mode <= 1) {
-    const float4 f = v.force[g];
-    p.x = fmaf(S.half_dt, f.x, p.x);
-    p.y = fmaf(S.half_dt, f.y, p.y);
-    p.z = fmaf(S.half_dt, f.z, p.z);
-    float4 x = v.pos[g];
-    const float hv = S.half_dt * p.w;
-    x.x = fmaf(hv, p.x, x.x);
-    x.y = fmaf(hv, p.y, x.y);
-    x.z = fmaf(hv, p.z, x.z);
-    const float3 xi = normal3(S.seed, step, g - S.a0, 0u);
-    const float amp = S.c2 * S.sqrt_kT * v.sqrt_m[g];
-    p.x = fmaf(S.c1, p.x, amp * xi.x);
-    p.y = fmaf(S.c1, p.y, amp * xi.y);
-    p.z = fmaf(S.c1, p.z, amp * xi.z);
-    x.x = fmaf(hv, p.x, x.x);
-    x.y = fmaf(hv, p.y, x.y);
-    x.z = fmaf(hv, p.z, x.z);
-    v.pos[g] = x;
-    v.acc[(size_t)g * 4 + 0] = 0ull;
-    v.acc[(size_t)g * 4 + 1] = 0ull;
-    v.acc[(size_t)g * 4 + 2] = 0ull;
-    v.acc[(size_t)g * 4 + 3] = 0ull;
-    v.iacc[g] = 0ull;
-  }
+  float4 p = v.mom[g];
+  begin_step(v, S, s, g, p, v.atom_step[g]);
   v.mom[g] = p;
 }
 
@@ -222,7 +301,7 @@ __global__ __launch_bounds__(kRedThreads) void k_energy_reduce(DeviceView v, con
   double e_dih = 0, e_imp = 0;
   for (int i = R[8] + t; i < R[9]; i += kRedThreads) {
     if (v.tors[i].idx.x < 0) continue;
-    if (v.tors[i].p.w == 0.0f) e_dih += v.e_tors[i]; else e_imp += v.e_tors[i];
+    if (v.tors[i].a.w == 0.0f) e_dih += v.e_tors[i]; else e_imp += v.e_tors[i];
   }
   double e_cm = 0;
   for (int i = R[10] + t; i < R[11]; i += kRedThreads)
@@ -325,24 +404,23 @@ __global__ void k_born_out(DeviceView v, float* out, const int32_t* map, int n) 
 
 static inline dim3 atom_grid(int n) { return dim3((n + 255) / 256); }
 
-void launch_zero_acc(const DeviceView& v, hipStream_t s) {
-  if (v.atom_hw) hipLaunchKernelGGL(k_zero_acc, atom_grid(v.atom_hw), dim3(256), 0, s, v);
+static inline dim3 tile_grid(int n_tiles) { return dim3((n_tiles + 3) / 4); }
+
+void launch_born_gather(const DeviceView& v, hipStream_t s) {
+  if (v.n_tiles) hipLaunchKernelGGL(k_born_gather, tile_grid(v.n_tiles), dim3(256), 0, s, v);
 }
-void launch_born_finish(const DeviceView& v, hipStream_t s) {
-  if (v.atom_hw) hipLaunchKernelGGL(k_born_finish, atom_grid(v.atom_hw), dim3(256), 0, s, v);
-}
-void launch_chain_prep(const DeviceView& v, bool energy, hipStream_t s) {
-  if (!v.atom_hw) return;
+void launch_force_gather(const DeviceView& v, bool energy, hipStream_t s) {
+  if (!v.n_tiles) return;
   if (energy)
-    hipLaunchKernelGGL(k_chain_prep<true>, atom_grid(v.atom_hw), dim3(256), 0, s, v);
+    hipLaunchKernelGGL(k_force_gather<true>, tile_grid(v.n_tiles), dim3(256), 0, s, v);
   else
-    hipLaunchKernelGGL(k_chain_prep<false>, atom_grid(v.atom_hw), dim3(256), 0, s, v);
+    hipLaunchKernelGGL(k_force_gather<false>, tile_grid(v.n_tiles), dim3(256), 0, s, v);
 }
-void launch_gather_forces(const DeviceView& v, hipStream_t s) {
-  if (v.atom_hw) hipLaunchKernelGGL(k_gather_forces, atom_grid(v.atom_hw), dim3(256), 0, s, v);
+void launch_chain_gather(const DeviceView& v, int mode, hipStream_t s) {
+  if (v.n_tiles) hipLaunchKernelGGL(k_chain_gather, tile_grid(v.n_tiles), dim3(256), 0, s, v, mode);
 }
-void launch_kick_drift(const DeviceView& v, int mode, hipStream_t s) {
-  if (v.atom_hw) hipLaunchKernelGGL(k_kick_drift, atom_grid(v.atom_hw), dim3(256), 0, s, v, mode);
+void launch_step_begin(const DeviceView& v, hipStream_t s) {
+  if (v.atom_hw) hipLaunchKernelGGL(k_step_begin, atom_grid(v.atom_hw), dim3(256), 0, s, v);
 }
 void launch_energy_reduce(const DeviceView& v, const int32_t* ranges, int n_slots, double* out,
                           hipStream_t s) {

@@ -1,5 +1,8 @@
-// Bonded terms: one thread per arena entry, forces scattered into the same
-// 64-bit fixed-point accumulators as the pair passes.
+// Bonded terms: one thread per arena entry. Each entry writes the force on
+// each of its atoms to its own float4 record (no atomics); every atom later
+// sums its records in a fixed order through a per-atom list (see
+// gather_bonded in prx_atom.hip), so the result is deterministic and free of
+// same-address atomic contention between neighbouring terms.
 //
 //   bonds / Urey-Bradley  1/2 k (r - r0)^2             bonded.py:20-44
 //   angles                1/2 k (theta - theta0)^2     bonded.py:47-81
@@ -15,16 +18,6 @@ namespace prx {
 
 constexpr int kBondedThreads = 256;
 
-__device__ __forceinline__ void scatter3(const DeviceView& v, int g, float fx, float fy, float fz) {
-  unsigned long long* a = &v.acc[(size_t)g * 4];
-  const long long qx = __double2ll_rn((double)fx * kForceScale);
-  const long long qy = __double2ll_rn((double)fy * kForceScale);
-  const long long qz = __double2ll_rn((double)fz * kForceScale);
-  if (qx) atomicAdd(a + 0, (unsigned long long)qx);
-  if (qy) atomicAdd(a + 1, (unsigned long long)qy);
-  if (qz) atomicAdd(a + 2, (unsigned long long)qz);
-}
-
 __device__ __forceinline__ float3 ld3(const DeviceView& v, int g) {
   const float4 p = v.pos[g];
   return make_float3(p.x, p.y, p.z);
@@ -36,6 +29,7 @@ __device__ __forceinline__ float3 cross3(float3 a, float3 b) {
 }
 __device__ __forceinline__ float3 scl3(float s, float3 a) { return make_float3(s * a.x, s * a.y, s * a.z); }
 __device__ __forceinline__ float3 add3(float3 a, float3 b) { return make_float3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ void put(float4* r, float3 f) { *r = make_float4(f.x, f.y, f.z, 0.0f); }
 
 // Dihedral angle as bonded.py:84-107 (b1 norm + 1e-8) and its gradient with
 // respect to the four atoms (Blondel & Karplus, J. Comput. Chem. 17, 1132).
@@ -66,11 +60,11 @@ __device__ __forceinline__ Dihedral dihedral(float3 p0, float3 p1, float3 p2, fl
   return d;
 }
 
-__device__ __forceinline__ void scatter_torsion(const DeviceView& v, int4 a, const Dihedral& d, float dEdphi) {
-  scatter3(v, a.x, -dEdphi * d.g0.x, -dEdphi * d.g0.y, -dEdphi * d.g0.z);
-  scatter3(v, a.y, -dEdphi * d.g1.x, -dEdphi * d.g1.y, -dEdphi * d.g1.z);
-  scatter3(v, a.z, -dEdphi * d.g2.x, -dEdphi * d.g2.y, -dEdphi * d.g2.z);
-  scatter3(v, a.w, -dEdphi * d.g3.x, -dEdphi * d.g3.y, -dEdphi * d.g3.z);
+__device__ __forceinline__ void torsion_records(float4* r, const Dihedral& d, float dEdphi) {
+  put(r + 0, scl3(-dEdphi, d.g0));
+  put(r + 1, scl3(-dEdphi, d.g1));
+  put(r + 2, scl3(-dEdphi, d.g2));
+  put(r + 3, scl3(-dEdphi, d.g3));
 }
 
 template <bool ENERGY>
@@ -81,8 +75,9 @@ __device__ void do_bond(const DeviceView& v, int i) {
   const float r = sqrtf(dot3(d, d));
   const float dr = r - b.p.x;
   const float s = -b.p.y * dr / r;  // F_0 = s * d
-  scatter3(v, b.idx.x, s * d.x, s * d.y, s * d.z);
-  scatter3(v, b.idx.y, -s * d.x, -s * d.y, -s * d.z);
+  float4* rec = v.rec + v.rec_base[0] + (int64_t)i * kRecBond;
+  put(rec + 0, scl3(s, d));
+  put(rec + 1, scl3(-s, d));
   if (ENERGY) v.e_bond[i] = 0.5 * (double)b.p.y * (double)dr * (double)dr;
 }
 
@@ -104,10 +99,18 @@ __device__ void do_angle(const DeviceView& v, int i) {
   const float k1 = P / (Q * Q);
   const float3 g0 = sub3(scl3(1.0f / Q, v2), scl3(k1 * d2 / d1, v1));
   const float3 g2 = sub3(scl3(1.0f / Q, v1), scl3(k1 * d1 / d2, v2));
-  scatter3(v, a.idx.x, -dEdc * g0.x, -dEdc * g0.y, -dEdc * g0.z);
-  scatter3(v, a.idx.z, -dEdc * g2.x, -dEdc * g2.y, -dEdc * g2.z);
-  scatter3(v, a.idx.y, dEdc * (g0.x + g2.x), dEdc * (g0.y + g2.y), dEdc * (g0.z + g2.z));
+  float4* rec = v.rec + v.rec_base[1] + (int64_t)i * kRecAngle;
+  put(rec + 0, scl3(-dEdc, g0));
+  put(rec + 1, scl3(dEdc, add3(g0, g2)));
+  put(rec + 2, scl3(-dEdc, g2));
   if (ENERGY) v.e_angle[i] = 0.5 * (double)a.p.y * (double)dth * (double)dth;
+}
+
+__device__ __forceinline__ void periodic_term(float n, float phase, float k, float phi, float& e, float& dE) {
+  float sa, ca;
+  sincosf(n * phi - phase, &sa, &ca);
+  e += k * (1.0f + ca);
+  dE -= k * n * sa;
 }
 
 template <bool ENERGY>
@@ -115,21 +118,18 @@ __device__ void do_torsion(const DeviceView& v, int i) {
   const TorsionEntry t = v.tors[i];
   if (t.idx.x < 0) return;
   const Dihedral d = dihedral(ld3(v, t.idx.x), ld3(v, t.idx.y), ld3(v, t.idx.z), ld3(v, t.idx.w));
-  float e, dE;
-  if (t.p.w == 2.0f) {  // harmonic improper: p.x = k, p.y = phi0
+  float e = 0.0f, dE = 0.0f;
+  if (t.a.w == 2.0f) {  // harmonic improper: a.x = k, a.y = phi0
     const float twopi = 6.283185307179586f;
-    float diff = d.phi - t.p.y + 3.141592653589793f;
+    float diff = d.phi - t.a.y + 3.141592653589793f;
     diff = diff - twopi * floorf(diff / twopi) - 3.141592653589793f;
-    e = t.p.x * diff * diff;
-    dE = 2.0f * t.p.x * diff;
+    e = t.a.x * diff * diff;
+    dE = 2.0f * t.a.x * diff;
   } else {
-    const float a = t.p.x * d.phi - t.p.y;
-    float sa, ca;
-    sincosf(a, &sa, &ca);
-    e = t.p.z * (1.0f + ca);
-    dE = -t.p.z * t.p.x * sa;
+    periodic_term(t.a.x, t.a.y, t.a.z, d.phi, e, dE);
+    if (t.b.w > 1.5f) periodic_term(t.b.x, t.b.y, t.b.z, d.phi, e, dE);
   }
-  scatter_torsion(v, t.idx, d, dE);
+  torsion_records(v.rec + v.rec_base[2] + (int64_t)i * kRecTors, d, dE);
   if (ENERGY) v.e_tors[i] = (double)e;
 }
 
@@ -164,9 +164,14 @@ __device__ void do_cmap(const DeviceView& v, int i) {
     e = e * da + rv;
     eb = eb * da + rd;
   }
-  const float dEa = ea;
-  scatter_torsion(v, c.idx, dp, dEa / delta);
-  scatter_torsion(v, make_int4(c.idx.y, c.idx.z, c.idx.w, c.idx2.x), ds, eb / delta);
+  const float dphi = ea / delta, dpsi = eb / delta;
+  // atoms a, b, c, d, e: phi uses a..d (gradients dp.g0..g3), psi uses b..e (ds.g0..g3)
+  float4* rec = v.rec + v.rec_base[3] + (int64_t)i * kRecCmap;
+  put(rec + 0, scl3(-dphi, dp.g0));
+  put(rec + 1, add3(scl3(-dphi, dp.g1), scl3(-dpsi, ds.g0)));
+  put(rec + 2, add3(scl3(-dphi, dp.g2), scl3(-dpsi, ds.g1)));
+  put(rec + 3, add3(scl3(-dphi, dp.g3), scl3(-dpsi, ds.g2)));
+  put(rec + 4, scl3(-dpsi, ds.g3));
   if (ENERGY) v.e_cmap[i] = (double)e;
 }
 
@@ -183,8 +188,9 @@ __device__ void do_exception(const DeviceView& v, int i) {
   const float e = 4.0f * x.p.y * (s6 * s6 - s6) + x.p.z / r;
   // -dE/dr / r
   const float fr = (4.0f * x.p.y * (12.0f * s6 * s6 - 6.0f * s6) + x.p.z / r) / r2;
-  scatter3(v, x.idx.x, fr * d.x, fr * d.y, fr * d.z);
-  scatter3(v, x.idx.y, -fr * d.x, -fr * d.y, -fr * d.z);
+  float4* rec = v.rec + v.rec_base[4] + (int64_t)i * kRecExc;
+  put(rec + 0, scl3(fr, d));
+  put(rec + 1, scl3(-fr, d));
   if (ENERGY) v.e_exc[i] = (double)e;
 }
 

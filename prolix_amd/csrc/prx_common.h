@@ -3,6 +3,11 @@
 // Every per-atom array is indexed by a global arena index g. A system ("slot")
 // owns the contiguous range [a0, a0 + n). Tiles never cross systems: tile k
 // of a system covers its atoms [a0 + 64k, a0 + 64k + 64).
+//
+// Reductions are deterministic by construction: every pair work item writes
+// its per-lane partial sums to its own slots of a partial buffer (no
+// atomics), and one wave per tile sums the partials of the tile's row and
+// column items in a fixed order.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -14,8 +19,6 @@ namespace prx {
 
 constexpr int kTile = 64;               // wave64: one i atom per lane, 64 j atoms rotate
 constexpr int kMaxExclClasses = 3;      // distinct non-unit (vdw, elec) scale pairs
-constexpr double kForceScale = 16777216.0;      // 2^24: fixed point of forces and dE/dB
-constexpr double kBornScale = 4294967296.0;     // 2^32: fixed point of the Born sums I_i
 constexpr float kLog2E = 1.4426950408889634f;
 constexpr float kLn2 = 0.6931471805599453f;
 
@@ -41,7 +44,7 @@ __host__ __device__ inline int item_slot(int32_t m) { return (int)((uint32_t)m >
 struct SlotConst {
   int32_t a0, n;        // atom range
   int32_t active;       // 1 if stepping
-  int32_t pad0;
+  int32_t item_base;    // index of the system's first work item (row-major I <= J)
   float cutoff2;        // 0 = no cutoff
   float k_coulomb;      // Coulomb constant
   float gb2;            // 2 * GB prefactor, prefactor = -1/2 k_gb (1/eps_in - 1/eps_out)
@@ -51,17 +54,31 @@ struct SlotConst {
   float excl_e[4];      // class -> elec scale
   // integrator (AKMA units)
   float dt, half_dt, c1, c2;   // c1 = exp(-gamma dt), c2 = sqrt(1 - c1^2)
-  float sqrt_kT, kT, pad2, pad3;
+  float sqrt_kT, kT;
+  int32_t caps;         // 1: batched-step safety caps (velocity, displacement, force)
+  int32_t pad2;
   uint64_t seed;
 };
+
+// Safety caps of prolix's batched BAOAB step (batched_simulate.py:269-358), in
+// the engine's AKMA units: |p/m| <= 20 A/tau, |dx| <= 0.5 A per step,
+// |F| <= 1e4 kcal/mol/A per atom.
+constexpr float kVLimit = 20.0f;
+constexpr float kMaxDx = 0.5f;
+constexpr float kForceCap = 10000.0f;
+enum { kWarnVLimit = 0, kWarnForceCap = 1, kWarnConstr = 2, kWarnDxCap = 3 };
 
 // Bonded entries; idx.x < 0 marks a free arena slot.
 struct BondEntry { int4 idx; float4 p; };    // idx.xy atoms, idx.z kind (0 bond, 1 UB); p.xy = r0, k
 struct AngleEntry { int4 idx; float4 p; };   // idx.xyz atoms; p.xy = theta0, k
-struct TorsionEntry { int4 idx; float4 p; }; // p = (n, phase, k, kind) kind 0 dihedral, 1 periodic
-                                             // improper, 2 harmonic improper (p.x = k, p.y = phi0)
+// Up to two terms of one torsion: a = (n0, phase0, k0, kind), b = (n1, phase1, k1, n_terms).
+// kind 0 dihedral, 1 periodic improper, 2 harmonic improper (a = (k, phi0, 0, 2), one term).
+struct TorsionEntry { int4 idx; float4 a; float4 b; };
 struct CmapEntry { int4 idx; int4 idx2; };   // idx = atoms a..d, idx2.x = atom e, .y = coeff base, .z = G
 struct ExceptionEntry { int4 idx; float4 p; };  // idx.xy pair; p = (sigma, eps, qq * k_coulomb, 0)
+
+// Atoms per bonded entry kind; each entry writes one float4 force record per atom.
+constexpr int kRecBond = 2, kRecAngle = 3, kRecTors = 4, kRecCmap = 5, kRecExc = 2;
 
 // All device pointers of one engine.
 struct DeviceView {
@@ -75,13 +92,15 @@ struct DeviceView {
   float* sqrt_m;       // sqrt(mass)                  (static)
   int32_t* slot_of;    // slot id or -1
   float4* gb4;         // B, exp factor a = sqrt(log2(e)/4)/B, dB/dI, chain factor c
-  unsigned long long* acc;   // 4 per atom: fx, fy, fz, dE/dB (fixed point 2^24)
-  unsigned long long* iacc;  // Born sums (fixed point 2^32)
+  float4* fpair;       // GB + Coulomb + LJ force of the last force pass
   double* e_atom;      // 2 per atom: GB self, ACE
   int32_t atom_hw;     // arena high-water mark
   // pair work
   const PairItem* items;
   int32_t n_items;
+  float4* part;        // partial sums: item w, side s (0 = i, 1 = j), lane l -> part[(w * 2 + s) * 64 + l]
+  const int4* tiles;   // per tile: (first atom, atoms in tile, slot, tile index T)
+  int32_t n_tiles;
   const unsigned long long* excl_masks;  // 128 per exclusion tile: lane * 2 + {bit0, bit1}
   double* e_items;     // 3 per item: GB pairs, Coulomb, LJ
   // bonded
@@ -92,22 +111,29 @@ struct DeviceView {
   ExceptionEntry* excs; int32_t exc_hw;
   const float* cmap_coeff;
   double* e_bond; double* e_angle; double* e_tors; double* e_cmap; double* e_exc;
+  // bonded force records and their per-atom gather lists
+  float4* rec;             // records; kind k entry e atom a -> rec_base[k] + e * atoms(k) + a
+  int64_t rec_base[5];     // bond, angle, torsion, cmap, exception
+  int2* atom_rec;          // per atom: (offset into rec_list, count)
+  const int32_t* rec_list; // record indices
   // systems
   SlotConst* slots;
   uint32_t* atom_step;     // per atom: BAOAB steps taken (the noise counter)
+  unsigned int* warn;      // per slot: 4 safety-cap event counters (atom-steps)
   int32_t max_slots;
 };
 
 // Launchers (defined in the .hip translation units).
 void launch_born_pairs(const DeviceView& v, hipStream_t s);
-void launch_born_finish(const DeviceView& v, hipStream_t s);
+void launch_born_gather(const DeviceView& v, hipStream_t s);               // -> B, dB/dI
 void launch_force_pairs(const DeviceView& v, bool energy, hipStream_t s);
-void launch_chain_prep(const DeviceView& v, bool energy, hipStream_t s);
+void launch_force_gather(const DeviceView& v, bool energy, hipStream_t s);  // -> chain factors, pair force
 void launch_chain_pairs(const DeviceView& v, hipStream_t s);
 void launch_bonded(const DeviceView& v, bool energy, hipStream_t s);
-void launch_zero_acc(const DeviceView& v, hipStream_t s);
-void launch_kick_drift(const DeviceView& v, int mode, hipStream_t s);  // 0: first BAOA, 1: B + BAOA, 2: final B
-void launch_gather_forces(const DeviceView& v, hipStream_t s);
+// Sum chain partials + pair force + bonded records into the force, then
+//   mode -1: store the force only; 2: B(F), count the step; 1: B(F) then B A O A.
+void launch_chain_gather(const DeviceView& v, int mode, hipStream_t s);
+void launch_step_begin(const DeviceView& v, hipStream_t s);  // B A O A from the stored force
 void launch_energy_reduce(const DeviceView& v, const int32_t* slot_ranges, int n_slots, double* out,
                           hipStream_t s);
 void launch_copy_in_positions(const DeviceView& v, const float* packed, const int32_t* map, int n,

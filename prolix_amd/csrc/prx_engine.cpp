@@ -98,24 +98,69 @@ struct Slot {
   bool live = false;
   int32_t a0 = 0, n = 0;
   int64_t bond_b = 0, bond_n = 0, ang_b = 0, ang_n = 0, tors_b = 0, tors_n = 0, cmap_b = 0, cmap_n = 0;
-  int64_t exc_b = 0, exc_n = 0, coeff_b = 0, coeff_n = 0, xt_b = 0, xt_n = 0;
+  int64_t exc_b = 0, exc_n = 0, coeff_b = 0, coeff_n = 0, xt_b = 0, xt_n = 0, rl_b = 0, rl_n = 0;
   std::vector<prx::PairItem> items;  // global indices, fixed for the slot's lifetime
   prx::SlotConst c{};
   bool has_params = false;
 };
 
+// Torsion entries of one system: the nonzero periodic terms of each torsion
+// in groups of two; one entry per nonzero harmonic improper term.
+void torsion_entries(const prx_system_desc& d, std::vector<prx::TorsionEntry>* out, int64_t* count,
+                     int a0 = 0) {
+  int64_t n = 0;
+  auto emit_periodic = [&](const int32_t* idx, const double* prm, int T, float kind) {
+    std::vector<const double*> terms;
+    for (int k = 0; k < T; ++k)
+      if (prm[(size_t)k * 3 + 2] != 0.0) terms.push_back(prm + (size_t)k * 3);
+    for (size_t k = 0; k < terms.size(); k += 2) {
+      ++n;
+      if (!out) continue;
+      prx::TorsionEntry e;
+      e.idx = make_int4(a0 + idx[0], a0 + idx[1], a0 + idx[2], a0 + idx[3]);
+      e.a = make_float4((float)terms[k][0], (float)terms[k][1], (float)terms[k][2], kind);
+      if (k + 1 < terms.size())
+        e.b = make_float4((float)terms[k + 1][0], (float)terms[k + 1][1], (float)terms[k + 1][2], 2.0f);
+      else
+        e.b = make_float4(0.f, 0.f, 0.f, 1.0f);
+      out->push_back(e);
+    }
+  };
+  for (int t = 0; t < d.n_dihedrals; ++t)
+    emit_periodic(&d.dihedral_idx[4 * t], &d.dihedral_params[(size_t)t * d.dihedral_terms * 3], d.dihedral_terms,
+                  0.0f);
+  const int w = d.improper_param_width;
+  for (int t = 0; t < d.n_impropers; ++t) {
+    const double* prm = &d.improper_params[(size_t)t * d.improper_terms * w];
+    if (w == 3) {
+      emit_periodic(&d.improper_idx[4 * t], prm, d.improper_terms, 1.0f);
+      continue;
+    }
+    for (int k = 0; k < d.improper_terms; ++k) {
+      if (prm[(size_t)k * 2] == 0.0) continue;
+      ++n;
+      if (!out) continue;
+      prx::TorsionEntry e;
+      const int32_t* idx = &d.improper_idx[4 * t];
+      e.idx = make_int4(a0 + idx[0], a0 + idx[1], a0 + idx[2], a0 + idx[3]);
+      e.a = make_float4((float)prm[(size_t)k * 2], (float)prm[(size_t)k * 2 + 1], 0.0f, 2.0f);
+      e.b = make_float4(0.f, 0.f, 0.f, 1.0f);
+      out->push_back(e);
+    }
+  }
+  if (count) *count = n;
+}
+
 int64_t count_torsion_entries(const prx_system_desc& d) {
   int64_t n = 0;
-  for (int t = 0; t < d.n_dihedrals; ++t)
-    for (int k = 0; k < d.dihedral_terms; ++k)
-      if (d.dihedral_params[((size_t)t * d.dihedral_terms + k) * 3 + 2] != 0.0) ++n;
-  const int w = d.improper_param_width;
-  for (int t = 0; t < d.n_impropers; ++t)
-    for (int k = 0; k < d.improper_terms; ++k) {
-      const double kk = d.improper_params[((size_t)t * d.improper_terms + k) * w + (w == 2 ? 0 : 2)];
-      if (kk != 0.0) ++n;
-    }
+  torsion_entries(d, nullptr, &n);
   return n;
+}
+
+int64_t count_records(const prx_system_desc& d) {
+  return (int64_t)prx::kRecBond * (d.n_bonds + d.n_ub) + (int64_t)prx::kRecAngle * d.n_angles +
+         (int64_t)prx::kRecTors * count_torsion_entries(d) + (int64_t)prx::kRecCmap * d.n_cmap +
+         (int64_t)prx::kRecExc * d.n_exceptions;
 }
 
 void check_idx(const int32_t* idx, int64_t rows, int cols, int n, const char* what) {
@@ -162,15 +207,19 @@ void validate(const prx_system_desc& d) {
 struct prx_engine {
   // capacities
   int64_t cap_atoms = 0, cap_bond = 0, cap_ang = 0, cap_tors = 0, cap_cmap = 0, cap_exc = 0;
-  int64_t cap_coeff = 0, cap_xt = 0;
+  int64_t cap_coeff = 0, cap_xt = 0, cap_rl = 0;
   int32_t cap_slots = 0;
   bool pool = false;
-  RangeAlloc a_atoms, a_bond, a_ang, a_tors, a_cmap, a_exc, a_coeff, a_xt;
+  RangeAlloc a_atoms, a_bond, a_ang, a_tors, a_cmap, a_exc, a_coeff, a_xt, a_rl;
+  int32_t* d_rec_list = nullptr;
   std::vector<Slot> slots;
   prx::DeviceView v{};
   // device buffers owned here
   prx::PairItem* d_items = nullptr;
   int64_t cap_items = 0;
+  float4* d_part = nullptr;   // 2 * 64 partials per item
+  int4* d_tiles = nullptr;
+  int64_t cap_tiles = 0;
   double* d_e_items = nullptr;
   unsigned long long* d_excl = nullptr;
   float* d_coeff = nullptr;
@@ -197,8 +246,7 @@ struct prx_engine {
     v.slot_of = dalloc<int32_t>(cap_atoms);
     HIPCHECK(hipMemset(v.slot_of, 0xFF, std::max<int64_t>(cap_atoms, 1) * sizeof(int32_t)));
     v.gb4 = dalloc<float4>(cap_atoms);
-    v.acc = dalloc<unsigned long long>(cap_atoms * 4);
-    v.iacc = dalloc<unsigned long long>(cap_atoms);
+    v.fpair = dalloc<float4>(cap_atoms);
     v.e_atom = dalloc<double>(cap_atoms * 2);
     v.atom_step = dalloc<uint32_t>(cap_atoms);
     v.bonds = dalloc<prx::BondEntry>(cap_bond);
@@ -212,6 +260,18 @@ struct prx_engine {
     v.e_tors = dalloc<double>(cap_tors);
     v.e_cmap = dalloc<double>(cap_cmap);
     v.e_exc = dalloc<double>(cap_exc);
+    v.rec_base[0] = 0;
+    v.rec_base[1] = v.rec_base[0] + (int64_t)prx::kRecBond * cap_bond;
+    v.rec_base[2] = v.rec_base[1] + (int64_t)prx::kRecAngle * cap_ang;
+    v.rec_base[3] = v.rec_base[2] + (int64_t)prx::kRecTors * cap_tors;
+    v.rec_base[4] = v.rec_base[3] + (int64_t)prx::kRecCmap * cap_cmap;
+    const int64_t n_rec = v.rec_base[4] + (int64_t)prx::kRecExc * cap_exc;
+    v.rec = dalloc<float4>(n_rec);
+    v.atom_rec = dalloc<int2>(cap_atoms);
+    cap_rl = std::max<int64_t>(cap_rl, 1);
+    d_rec_list = dalloc<int32_t>(cap_rl);
+    v.rec_list = d_rec_list;
+    v.warn = dalloc<unsigned int>((size_t)cap_slots * 4);
     d_coeff = dalloc<float>(cap_coeff);
     v.cmap_coeff = d_coeff;
     d_excl = dalloc<unsigned long long>(cap_xt * 128);
@@ -244,10 +304,10 @@ struct prx_engine {
 
   void release_device() {
     drop_graphs();
-    void* ptrs[] = {v.pos, v.mom, v.force, v.born4, v.nb4, v.fin4, v.sqrt_m, v.slot_of, v.gb4, v.acc, v.iacc,
+    void* ptrs[] = {v.pos, v.mom, v.force, v.born4, v.nb4, v.fin4, v.sqrt_m, v.slot_of, v.gb4, v.fpair, d_part, d_tiles,
                     v.e_atom, v.atom_step, v.bonds, v.angles, v.tors, v.cmaps, v.excs, v.e_bond, v.e_angle,
                     v.e_tors, v.e_cmap, v.e_exc, d_coeff, d_excl, v.slots, d_ranges, d_slot_list,
-                    d_energy_tmp, d_map, d_items, d_e_items};
+                    d_energy_tmp, d_map, d_items, d_e_items, v.rec, v.atom_rec, d_rec_list, v.warn};
     for (void* p : ptrs)
       if (p) hipFree(p);
     if (stream) hipStreamDestroy(stream);
@@ -277,11 +337,12 @@ struct prx_engine {
     const int64_t n_cmap = d.n_cmap, n_exc = d.n_exceptions;
     const int64_t n_coeff = d.n_cmap ? (int64_t)d.n_cmap_maps * d.cmap_grid * d.cmap_grid * 16 : 0;
     const int64_t n_xt = (int64_t)plan.tiles.size();
+    const int64_t n_rl = count_records(d);
     struct Want { RangeAlloc* a; int64_t n; int64_t* b; };
     int64_t a0 = 0;
     Want w[] = {{&a_atoms, s.n, &a0}, {&a_bond, n_bond, &s.bond_b}, {&a_ang, n_ang, &s.ang_b},
                 {&a_tors, n_tors, &s.tors_b}, {&a_cmap, n_cmap, &s.cmap_b}, {&a_exc, n_exc, &s.exc_b},
-                {&a_coeff, n_coeff, &s.coeff_b}, {&a_xt, n_xt, &s.xt_b}};
+                {&a_coeff, n_coeff, &s.coeff_b}, {&a_xt, n_xt, &s.xt_b}, {&a_rl, n_rl, &s.rl_b}};
     int got = 0;
     for (auto& x : w) {
       const int64_t b = x.a->alloc(x.n);
@@ -294,7 +355,7 @@ struct prx_engine {
     }
     s.a0 = (int32_t)a0;
     s.bond_n = n_bond; s.ang_n = n_ang; s.tors_n = n_tors; s.cmap_n = n_cmap; s.exc_n = n_exc;
-    s.coeff_n = n_coeff; s.xt_n = n_xt;
+    s.coeff_n = n_coeff; s.xt_n = n_xt; s.rl_n = n_rl;
     upload(d, sid, s, plan);
     prx::build_items(s.n, s.a0, sid, plan, (int)s.xt_b, s.items);
     s.live = true;
@@ -365,32 +426,7 @@ struct prx_engine {
     if (s.tors_n) {
       std::vector<prx::TorsionEntry> e;
       e.reserve((size_t)s.tors_n);
-      for (int t = 0; t < d.n_dihedrals; ++t)
-        for (int k = 0; k < d.dihedral_terms; ++k) {
-          const double* p = &d.dihedral_params[((size_t)t * d.dihedral_terms + k) * 3];
-          if (p[2] == 0.0) continue;
-          prx::TorsionEntry x;
-          x.idx = make_int4(a0 + d.dihedral_idx[4 * t], a0 + d.dihedral_idx[4 * t + 1],
-                            a0 + d.dihedral_idx[4 * t + 2], a0 + d.dihedral_idx[4 * t + 3]);
-          x.p = make_float4((float)p[0], (float)p[1], (float)p[2], 0.0f);
-          e.push_back(x);
-        }
-      const int w = d.improper_param_width;
-      for (int t = 0; t < d.n_impropers; ++t)
-        for (int k = 0; k < d.improper_terms; ++k) {
-          const double* p = &d.improper_params[((size_t)t * d.improper_terms + k) * w];
-          prx::TorsionEntry x;
-          x.idx = make_int4(a0 + d.improper_idx[4 * t], a0 + d.improper_idx[4 * t + 1],
-                            a0 + d.improper_idx[4 * t + 2], a0 + d.improper_idx[4 * t + 3]);
-          if (w == 2) {
-            if (p[0] == 0.0) continue;
-            x.p = make_float4((float)p[0], (float)p[1], 0.0f, 2.0f);
-          } else {
-            if (p[2] == 0.0) continue;
-            x.p = make_float4((float)p[0], (float)p[1], (float)p[2], 1.0f);
-          }
-          e.push_back(x);
-        }
+      torsion_entries(d, &e, nullptr, a0);
       HIPCHECK(hipMemcpy(v.tors + s.tors_b, e.data(), e.size() * sizeof(e[0]), hipMemcpyHostToDevice));
     }
     if (s.cmap_n) {
@@ -422,6 +458,7 @@ struct prx_engine {
     if (s.xt_n)
       HIPCHECK(hipMemcpy(d_excl + s.xt_b * 128, plan.tile_masks.data(), plan.tile_masks.size() * 8,
                          hipMemcpyHostToDevice));
+    upload_record_lists(d, s);
     // constants
     prx::SlotConst& c = s.c;
     std::memset(&c, 0, sizeof(c));
@@ -443,6 +480,42 @@ struct prx_engine {
     push_const(sid, c);
   }
 
+  // Per-atom lists of the bonded force records that land on each atom, in a
+  // fixed order (entry kind, then entry, then position in the entry).
+  void upload_record_lists(const prx_system_desc& d, const Slot& s) {
+    const int n = s.n;
+    std::vector<std::vector<int32_t>> lists(n);
+    auto add = [&](int local_atom, int64_t rec) { lists[local_atom].push_back((int32_t)rec); };
+    int64_t k = 0;
+    for (int b = 0; b < d.n_bonds; ++b, ++k)
+      for (int a = 0; a < 2; ++a) add(d.bond_idx[2 * b + a], v.rec_base[0] + (s.bond_b + k) * prx::kRecBond + a);
+    for (int b = 0; b < d.n_ub; ++b, ++k)
+      for (int a = 0; a < 2; ++a) add(d.ub_idx[2 * b + a], v.rec_base[0] + (s.bond_b + k) * prx::kRecBond + a);
+    for (int b = 0; b < d.n_angles; ++b)
+      for (int a = 0; a < 3; ++a) add(d.angle_idx[3 * b + a], v.rec_base[1] + (s.ang_b + b) * prx::kRecAngle + a);
+    std::vector<prx::TorsionEntry> te;
+    torsion_entries(d, &te, nullptr, 0);
+    for (size_t t = 0; t < te.size(); ++t) {
+      const int ids[4] = {te[t].idx.x, te[t].idx.y, te[t].idx.z, te[t].idx.w};
+      for (int a = 0; a < 4; ++a) add(ids[a], v.rec_base[2] + (s.tors_b + (int64_t)t) * prx::kRecTors + a);
+    }
+    for (int c = 0; c < d.n_cmap; ++c)
+      for (int a = 0; a < 5; ++a) add(d.cmap_torsions[5 * c + a], v.rec_base[3] + (s.cmap_b + c) * prx::kRecCmap + a);
+    for (int x = 0; x < d.n_exceptions; ++x)
+      for (int a = 0; a < 2; ++a) add(d.exc_pairs[2 * x + a], v.rec_base[4] + (s.exc_b + x) * prx::kRecExc + a);
+    std::vector<int2> ar(n);
+    std::vector<int32_t> flat;
+    flat.reserve((size_t)s.rl_n);
+    for (int i = 0; i < n; ++i) {
+      ar[i] = make_int2((int)(s.rl_b + (int64_t)flat.size()), (int)lists[i].size());
+      flat.insert(flat.end(), lists[i].begin(), lists[i].end());
+    }
+    if ((int64_t)flat.size() != s.rl_n) throw std::logic_error("bonded record count mismatch");
+    HIPCHECK(hipMemcpy(v.atom_rec + s.a0, ar.data(), n * sizeof(int2), hipMemcpyHostToDevice));
+    if (!flat.empty())
+      HIPCHECK(hipMemcpy(d_rec_list + s.rl_b, flat.data(), flat.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  }
+
   void push_const(int32_t sid, const prx::SlotConst& c) {
     HIPCHECK(hipMemcpy(v.slots + sid, &c, sizeof(c), hipMemcpyHostToDevice));
   }
@@ -461,7 +534,9 @@ struct prx_engine {
     c.c2 = (float)std::sqrt(std::max(0.0, 1.0 - c1 * c1));
     c.kT = (float)kT;
     c.sqrt_kT = (float)std::sqrt(kT);
+    c.caps = p.safety_caps ? 1 : 0;
     c.seed = p.seed;
+    HIPCHECK(hipMemset(v.warn + (size_t)sid * 4, 0, 4 * sizeof(unsigned int)));
     slots[sid].has_params = true;
     push_const(sid, c);
   }
@@ -470,6 +545,7 @@ struct prx_engine {
     if (sid < 0 || sid >= (int32_t)slots.size() || !slots[sid].live) throw std::invalid_argument("no such live slot");
     Slot& s = slots[sid];
     HIPCHECK(hipMemset(v.slot_of + s.a0, 0xFF, s.n * sizeof(int32_t)));
+    HIPCHECK(hipMemset(v.atom_rec + s.a0, 0, s.n * sizeof(int2)));
     free_entries(v.bonds, s.bond_b, s.bond_n);
     free_entries(v.angles, s.ang_b, s.ang_n);
     free_entries(v.tors, s.tors_b, s.tors_n);
@@ -483,6 +559,7 @@ struct prx_engine {
     a_exc.release(s.exc_b, s.exc_n);
     a_coeff.release(s.coeff_b, s.coeff_n);
     a_xt.release(s.xt_b, s.xt_n);
+    a_rl.release(s.rl_b, s.rl_n);
     s = Slot();
     order.erase(std::remove(order.begin(), order.end(), sid), order.end());
     dirty = true;
@@ -494,10 +571,16 @@ struct prx_engine {
     std::vector<prx::PairItem> items;
     std::vector<int32_t> ranges;
     std::vector<int32_t> map;
+    std::vector<int4> tiles;
     offsets.assign(1, 0);
     for (int32_t sid : order) {
-      const Slot& s = slots[sid];
+      Slot& s = slots[sid];
       const int32_t ib = (int32_t)items.size();
+      s.c.item_base = ib;
+      push_const(sid, s.c);
+      const int nT = (s.n + prx::kTile - 1) / prx::kTile;
+      for (int T = 0; T < nT; ++T)
+        tiles.push_back(make_int4(s.a0 + T * prx::kTile, std::min(prx::kTile, s.n - T * prx::kTile), sid, T));
       items.insert(items.end(), s.items.begin(), s.items.end());
       const int32_t r[14] = {ib, (int32_t)items.size(), s.a0, s.n, (int32_t)s.bond_b, (int32_t)(s.bond_b + s.bond_n),
                              (int32_t)s.ang_b, (int32_t)(s.ang_b + s.ang_n), (int32_t)s.tors_b,
@@ -510,10 +593,19 @@ struct prx_engine {
     if ((int64_t)items.size() > cap_items) {
       if (d_items) HIPCHECK(hipFree(d_items));
       if (d_e_items) HIPCHECK(hipFree(d_e_items));
+      if (d_part) HIPCHECK(hipFree(d_part));
       cap_items = std::max<int64_t>((int64_t)items.size(), cap_items * 3 / 2);
       d_items = dalloc<prx::PairItem>(cap_items);
       d_e_items = dalloc<double>(cap_items * 3);
+      d_part = dalloc<float4>(cap_items * 2 * prx::kTile);
     }
+    if ((int64_t)tiles.size() > cap_tiles) {
+      if (d_tiles) HIPCHECK(hipFree(d_tiles));
+      cap_tiles = std::max<int64_t>((int64_t)tiles.size(), cap_tiles * 3 / 2);
+      d_tiles = dalloc<int4>(cap_tiles);
+    }
+    if (!tiles.empty())
+      HIPCHECK(hipMemcpy(d_tiles, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice));
     if (!items.empty())
       HIPCHECK(hipMemcpy(d_items, items.data(), items.size() * sizeof(prx::PairItem), hipMemcpyHostToDevice));
     if (!ranges.empty())
@@ -522,6 +614,9 @@ struct prx_engine {
     v.items = d_items;
     v.n_items = (int32_t)items.size();
     v.e_items = d_e_items;
+    v.part = d_part;
+    v.tiles = d_tiles;
+    v.n_tiles = (int32_t)tiles.size();
     v.atom_hw = (int32_t)a_atoms.high_water();
     v.bond_hw = (int32_t)a_bond.high_water();
     v.angle_hw = (int32_t)a_ang.high_water();
@@ -535,19 +630,19 @@ struct prx_engine {
   int32_t total_atoms() const { return offsets.back(); }
 
   // ---- evaluation -------------------------------------------------------
+  // Everything of one force evaluation except the final chain gather.
   void force_passes(bool energy, hipStream_t s) {
     prx::launch_born_pairs(v, s);
-    prx::launch_born_finish(v, s);
+    prx::launch_born_gather(v, s);
     prx::launch_force_pairs(v, energy, s);
-    prx::launch_chain_prep(v, energy, s);
-    prx::launch_chain_pairs(v, s);
+    prx::launch_force_gather(v, energy, s);
     prx::launch_bonded(v, energy, s);
+    prx::launch_chain_pairs(v, s);
   }
 
   void evaluate(bool energy, hipStream_t s) {
-    prx::launch_zero_acc(v, s);
     force_passes(energy, s);
-    prx::launch_gather_forces(v, s);
+    prx::launch_chain_gather(v, -1, s);
   }
 
   void drop_graphs() {
@@ -561,10 +656,10 @@ struct prx_engine {
     hipGraphExec_t exec = nullptr;
     hipGraph_t g;
     HIPCHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
-    prx::launch_kick_drift(v, 0, stream);
+    prx::launch_step_begin(v, stream);
     for (int k = 0; k < n_steps; ++k) {
       force_passes(false, stream);
-      prx::launch_kick_drift(v, k + 1 == n_steps ? 2 : 1, stream);
+      prx::launch_chain_gather(v, k + 1 == n_steps ? 2 : 1, stream);
     }
     HIPCHECK(hipStreamEndCapture(stream, &g));
     HIPCHECK(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
@@ -653,6 +748,7 @@ int prx_system_create(const prx_system_desc* descs, int32_t n_systems, prx_engin
       eng->cap_exc += d.n_exceptions;
       eng->cap_coeff += d.n_cmap ? (int64_t)d.n_cmap_maps * d.cmap_grid * d.cmap_grid * 16 : 0;
       eng->cap_xt += (int64_t)plan.tiles.size();
+      eng->cap_rl += count_records(d);
     }
     eng->cap_slots = n_systems;
     eng->a_atoms = RangeAlloc(eng->cap_atoms);
@@ -663,6 +759,7 @@ int prx_system_create(const prx_system_desc* descs, int32_t n_systems, prx_engin
     eng->a_exc = RangeAlloc(eng->cap_exc);
     eng->a_coeff = RangeAlloc(eng->cap_coeff);
     eng->a_xt = RangeAlloc(eng->cap_xt);
+    eng->a_rl = RangeAlloc(std::max<int64_t>(eng->cap_rl, 1));
     try {
       eng->alloc_device();
       for (int k = 0; k < n_systems; ++k) eng->admit(descs[k]);
@@ -692,6 +789,10 @@ int prx_pool_create(const prx_pool_limits* lim, prx_engine** out) {
     eng->cap_coeff = std::max<int64_t>(lim->max_cmap_coeff, 1);
     eng->cap_xt = std::max<int64_t>(lim->max_excl_tiles, 1);
     eng->cap_slots = lim->max_systems;
+    eng->cap_rl = (int64_t)prx::kRecBond * eng->cap_bond + (int64_t)prx::kRecAngle * eng->cap_ang +
+                  (int64_t)prx::kRecTors * eng->cap_tors + (int64_t)prx::kRecCmap * eng->cap_cmap +
+                  (int64_t)prx::kRecExc * eng->cap_exc;
+    eng->a_rl = RangeAlloc(eng->cap_rl);
     eng->a_atoms = RangeAlloc(eng->cap_atoms);
     eng->a_bond = RangeAlloc(eng->cap_bond);
     eng->a_ang = RangeAlloc(eng->cap_ang);
@@ -809,6 +910,16 @@ int prx_integrator_step_count(prx_engine* eng, int32_t system, int64_t* steps) {
   });
 }
 
+int prx_integrator_warnings(prx_engine* eng, int32_t system, int64_t* counts) {
+  return guard([&] {
+    need(eng && counts && system >= 0 && system < (int32_t)eng->order.size(), "prx_integrator_warnings: bad arguments");
+    unsigned int w[4] = {0, 0, 0, 0};
+    HIPCHECK(hipDeviceSynchronize());
+    HIPCHECK(hipMemcpy(w, eng->v.warn + (size_t)eng->order[system] * 4, sizeof(w), hipMemcpyDeviceToHost));
+    for (int k = 0; k < 4; ++k) counts[k] = w[k];
+  });
+}
+
 int prx_pool_admit(prx_engine* pool, const prx_system_desc* desc, const float* positions_host,
                    const prx_langevin_params* params, int32_t* slot, void* stream) {
   return guard([&] {
@@ -901,6 +1012,53 @@ int prx_pool_energy(prx_engine* pool, int32_t slot, double* energies_host, void*
     const size_t row = (size_t)(it - pool->order.begin());
     HIPCHECK(hipMemcpy(energies_host, pool->d_energy_tmp + row * PRX_N_ENERGY, PRX_N_ENERGY * sizeof(double),
                        hipMemcpyDeviceToHost));
+  });
+}
+
+int prx_profile_passes(prx_engine* eng, int32_t n_reps, double* ms) {
+  return guard([&] {
+    need(eng && ms && n_reps > 0, "prx_profile_passes: bad arguments");
+    eng->rebuild();
+    HIPCHECK(hipDeviceSynchronize());
+    hipStream_t s = eng->stream;
+    const int n_pass = 7;
+    hipEvent_t ev[n_pass][2];
+    for (int k = 0; k < n_pass; ++k) {
+      HIPCHECK(hipEventCreate(&ev[k][0]));
+      HIPCHECK(hipEventCreate(&ev[k][1]));
+      ms[k] = 0.0;
+    }
+    for (int r = 0; r < n_reps; ++r) {
+      auto timed = [&](int k, auto&& launch) {
+        HIPCHECK(hipEventRecord(ev[k][0], s));
+        launch();
+        HIPCHECK(hipEventRecord(ev[k][1], s));
+        HIPCHECK(hipEventSynchronize(ev[k][1]));
+        float t = 0.f;
+        HIPCHECK(hipEventElapsedTime(&t, ev[k][0], ev[k][1]));
+        ms[k] += t;
+      };
+      timed(0, [&] { prx::launch_born_pairs(eng->v, s); });
+      timed(1, [&] { prx::launch_born_gather(eng->v, s); });
+      timed(2, [&] { prx::launch_force_pairs(eng->v, false, s); });
+      timed(3, [&] { prx::launch_force_gather(eng->v, false, s); });
+      timed(4, [&] { prx::launch_chain_pairs(eng->v, s); });
+      timed(5, [&] { prx::launch_bonded(eng->v, false, s); });
+      timed(6, [&] { prx::launch_chain_gather(eng->v, -1, s); });
+    }
+    for (int k = 0; k < n_pass; ++k) {
+      ms[k] /= n_reps;
+      hipEventDestroy(ev[k][0]);
+      hipEventDestroy(ev[k][1]);
+    }
+    HIPCHECK(hipGetLastError());
+  });
+}
+
+int prx_engine_work_items(const prx_engine* eng, int64_t* n_items) {
+  return guard([&] {
+    need(eng && n_items, "prx_engine_work_items: bad arguments");
+    *n_items = eng->v.n_items;
   });
 }
 

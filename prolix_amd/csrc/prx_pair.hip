@@ -4,9 +4,10 @@
 // owns i atom I0 + l; the 64 j atoms of tile J live one per lane in registers
 // and rotate one lane per iteration with a DPP wave rotate, together with
 // their force accumulators, so every lane meets every j exactly once and no
-// two lanes touch the same j in one iteration. After the loop each lane adds
-// its i and j partial sums to global 64-bit fixed-point accumulators: integer
-// adds commute, so results are bitwise reproducible in any launch order.
+// two lanes touch the same j in one iteration. After the loop the wave stores
+// its i and j partial sums in the item's own slots of the partial buffer
+// (plain coalesced stores, no atomics); the tile gathers of prx_atom.hip add
+// them up in a fixed order, so results are bitwise reproducible.
 //
 // Pass 1 (Born sums, generalized_born.py:211-263 with :32-94):
 //     I_i += I(r; rho_i, sr_j),  I_j += I(r; rho_j, sr_i)
@@ -19,6 +20,7 @@
 // Diagonal tiles (I == J) visit each unordered pair once: offsets k = 1..31
 // on every lane and k = 32 on lanes 0..31 only.
 #include "prx_common.h"
+#include "prx_obc_farfield.h"
 
 namespace prx {
 
@@ -31,9 +33,9 @@ __device__ __forceinline__ float rot(float x) {
   return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), PRX_DPP_WAVE_ROL1, 0xF, 0xF, true));
 }
 
-__device__ __forceinline__ void flush_fixed(unsigned long long* p, float x, double scale) {
-  long long q = __double2ll_rn((double)x * scale);
-  if (q != 0) atomicAdd(p, (unsigned long long)q);
+// Partial-buffer slot of (item w, side, lane); side 0 = i atoms, 1 = j atoms.
+__device__ __forceinline__ size_t part_slot(int w, int side, int lane) {
+  return ((size_t)w * 2 + side) * kTile + lane;
 }
 
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
@@ -106,12 +108,36 @@ __device__ __forceinline__ float born_dr(float r, float rs, float rho, float s, 
   return on * (d1 + d2 + d3 + d4);
 }
 
+// Far field (r - s > rho and s <= 0.6 r): L = r - s, U = r + s and
+//   I = x^3 h(x^2) / (2r),  dI/dr = -x^3 k(x^2) / r^2,  x = s / r,
+// with the polynomials of prx_obc_farfield.h. Same function as the direct
+// formula, without its cancellation and without a transcendental.
+__device__ __forceinline__ bool obc_far(float r, float rho, float s) {
+  return (r - s > rho) && (s <= PRX_OBC_FAR_XMAX * r);
+}
+
+__device__ __forceinline__ float obc_far_I(float s, float rinv) {
+  const float x = s * rinv, y = x * x;
+  float h = kObcH[kObcHDeg];
+#pragma unroll
+  for (int k = kObcHDeg - 1; k >= 0; --k) h = fmaf(h, y, kObcH[k]);
+  return 0.5f * x * y * rinv * h;
+}
+
+__device__ __forceinline__ float obc_far_dI(float s, float rinv) {
+  const float x = s * rinv, y = x * x;
+  float q = kObcK[kObcKDeg];
+#pragma unroll
+  for (int k = kObcKDeg - 1; k >= 0; --k) q = fmaf(q, y, kObcK[k]);
+  return -(x * y) * (rinv * rinv) * q;
+}
+
 // ---------------------------------------------------------------------------
 // Pass 1: Born sums
 // ---------------------------------------------------------------------------
 
 template <bool GENERAL, bool DIAG>
-__device__ __forceinline__ void born_tile(const DeviceView& v, const PairItem& it, int lane) {
+__device__ __forceinline__ void born_tile(const DeviceView& v, const PairItem& it, int lane, int w) {
   const int ni = item_ni(it.meta), nj = item_nj(it.meta);
   const int gi = it.i0 + min(lane, ni - 1);
   const int gj = it.j0 + min(lane, nj - 1);
@@ -130,7 +156,16 @@ __device__ __forceinline__ void born_tile(const DeviceView& v, const PairItem& i
     const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
     const float rinv = frsq(r2);
     const float r = r2 * rinv;
-    BornPair bp = born_pair(r, rinv, bi.x, bi.y, bi.z, rhoj, sj, sj2);
+    const bool far1 = obc_far(r, bi.x, sj), far2 = obc_far(r, rhoj, bi.y);
+    BornPair bp;
+    if (__all(far1 && far2)) {
+      bp.I1 = obc_far_I(sj, rinv);
+      bp.I2 = obc_far_I(bi.y, rinv);
+    } else {
+      bp = born_pair(r, rinv, bi.x, bi.y, bi.z, rhoj, sj, sj2);
+      bp.I1 = far1 ? obc_far_I(sj, rinv) : bp.I1;
+      bp.I2 = far2 ? obc_far_I(bi.y, rinv) : bp.I2;
+    }
     if (GENERAL) {
       const int orig = (lane + k) & 63;
       bool valid = ivalid && orig < nj;
@@ -147,8 +182,9 @@ __device__ __forceinline__ void born_tile(const DeviceView& v, const PairItem& i
   }
   const int nrot = DIAG ? 32 : 63;
   const int home = (lane + nrot) & 63;
-  if (ivalid) flush_fixed(&v.iacc[it.i0 + lane], accI, kBornScale);
-  if (home < nj) flush_fixed(&v.iacc[it.j0 + home], accJ, kBornScale);
+  float* pb = reinterpret_cast<float*>(v.part);  // scalar partials
+  pb[part_slot(w, 0, lane)] = accI;
+  pb[part_slot(w, 1, home)] = accJ;
 }
 
 __global__ __launch_bounds__(256) void k_born_pairs(DeviceView v) {
@@ -159,11 +195,11 @@ __global__ __launch_bounds__(256) void k_born_pairs(DeviceView v) {
   const int fl = item_flags(it.meta);
   const bool full = item_ni(it.meta) == kTile && item_nj(it.meta) == kTile;
   if (fl & kFlagDiag) {
-    born_tile<true, true>(v, it, lane);
+    born_tile<true, true>(v, it, lane, w);
   } else if (full) {
-    born_tile<false, false>(v, it, lane);
+    born_tile<false, false>(v, it, lane, w);
   } else {
-    born_tile<true, false>(v, it, lane);
+    born_tile<true, false>(v, it, lane, w);
   }
 }
 
@@ -266,20 +302,8 @@ __device__ __forceinline__ void force_tile(const DeviceView& v, const PairItem& 
   }
   const int nrot = DIAG ? 32 : 63;
   const int home = (lane + nrot) & 63;
-  if (ivalid) {
-    unsigned long long* a = &v.acc[(size_t)(it.i0 + lane) * 4];
-    flush_fixed(a + 0, fxi, kForceScale);
-    flush_fixed(a + 1, fyi, kForceScale);
-    flush_fixed(a + 2, fzi, kForceScale);
-    flush_fixed(a + 3, dbi, kForceScale);
-  }
-  if (home < nj) {
-    unsigned long long* a = &v.acc[(size_t)(it.j0 + home) * 4];
-    flush_fixed(a + 0, fxj, kForceScale);
-    flush_fixed(a + 1, fyj, kForceScale);
-    flush_fixed(a + 2, fzj, kForceScale);
-    flush_fixed(a + 3, dbj, kForceScale);
-  }
+  v.part[part_slot(w, 0, lane)] = make_float4(fxi, fyi, fzi, dbi);
+  v.part[part_slot(w, 1, home)] = make_float4(fxj, fyj, fzj, dbj);
   if (ENERGY) {
     for (int off = 32; off > 0; off >>= 1) {
       egb += __shfl_xor(egb, off);
@@ -317,7 +341,7 @@ __global__ __launch_bounds__(256) void k_force_pairs(DeviceView v) {
 // ---------------------------------------------------------------------------
 
 template <bool GENERAL, bool DIAG>
-__device__ __forceinline__ void chain_tile(const DeviceView& v, const PairItem& it, int lane) {
+__device__ __forceinline__ void chain_tile(const DeviceView& v, const PairItem& it, int lane, int w) {
   const int ni = item_ni(it.meta), nj = item_nj(it.meta);
   const int gi = it.i0 + min(lane, ni - 1);
   const int gj = it.j0 + min(lane, nj - 1);
@@ -337,9 +361,16 @@ __device__ __forceinline__ void chain_tile(const DeviceView& v, const PairItem& 
     const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
     const float rinv = frsq(r2);
     const float r = r2 * rinv;
-    const float rs = fminf(rinv, 1e4f);
-    const float d1 = born_dr(r, rs, bi.x, sj, sj2);     // dI(r; rho_i, s_j)/dr
-    const float d2 = born_dr(r, rs, rhoj, bi.y, bi.z);  // dI(r; rho_j, s_i)/dr
+    const bool far1 = obc_far(r, bi.x, sj), far2 = obc_far(r, rhoj, bi.y);
+    float d1, d2;  // dI(r; rho_i, s_j)/dr and dI(r; rho_j, s_i)/dr
+    if (__all(far1 && far2)) {
+      d1 = obc_far_dI(sj, rinv);
+      d2 = obc_far_dI(bi.y, rinv);
+    } else {
+      const float rs = fminf(rinv, 1e4f);
+      d1 = far1 ? obc_far_dI(sj, rinv) : born_dr(r, rs, bi.x, sj, sj2);
+      d2 = far2 ? obc_far_dI(bi.y, rinv) : born_dr(r, rs, rhoj, bi.y, bi.z);
+    }
     float gsc = fmaf(ci, d1, cj * d2) * rinv;           // F_i = gsc * (x_j - x_i)
     if (GENERAL) {
       const int orig = (lane + k) & 63;
@@ -357,18 +388,8 @@ __device__ __forceinline__ void chain_tile(const DeviceView& v, const PairItem& 
   }
   const int nrot = DIAG ? 32 : 63;
   const int home = (lane + nrot) & 63;
-  if (ivalid) {
-    unsigned long long* a = &v.acc[(size_t)(it.i0 + lane) * 4];
-    flush_fixed(a + 0, fxi, kForceScale);
-    flush_fixed(a + 1, fyi, kForceScale);
-    flush_fixed(a + 2, fzi, kForceScale);
-  }
-  if (home < nj) {
-    unsigned long long* a = &v.acc[(size_t)(it.j0 + home) * 4];
-    flush_fixed(a + 0, fxj, kForceScale);
-    flush_fixed(a + 1, fyj, kForceScale);
-    flush_fixed(a + 2, fzj, kForceScale);
-  }
+  v.part[part_slot(w, 0, lane)] = make_float4(fxi, fyi, fzi, 0.0f);
+  v.part[part_slot(w, 1, home)] = make_float4(fxj, fyj, fzj, 0.0f);
 }
 
 __global__ __launch_bounds__(256) void k_chain_pairs(DeviceView v) {
@@ -379,11 +400,11 @@ __global__ __launch_bounds__(256) void k_chain_pairs(DeviceView v) {
   const int fl = item_flags(it.meta);
   const bool full = item_ni(it.meta) == kTile && item_nj(it.meta) == kTile;
   if (fl & kFlagDiag) {
-    chain_tile<true, true>(v, it, lane);
+    chain_tile<true, true>(v, it, lane, w);
   } else if (full) {
-    chain_tile<false, false>(v, it, lane);
+    chain_tile<false, false>(v, it, lane, w);
   } else {
-    chain_tile<true, false>(v, it, lane);
+    chain_tile<true, false>(v, it, lane, w);
   }
 }
 

@@ -40,10 +40,11 @@ class Langevin:
     temperature: float = 300.0
     gamma_ps: float = 1.0
     seed: int = 0
+    safety_caps: bool = False  # the batched step's velocity / displacement / force caps
 
     def c(self) -> N.LangevinParams:
         return N.LangevinParams(float(self.dt_fs), float(self.temperature), float(self.gamma_ps),
-                                int(self.seed) & 0xFFFFFFFFFFFFFFFF)
+                                int(self.seed) & 0xFFFFFFFFFFFFFFFF, 1 if self.safety_caps else 0, 0)
 
 
 class GBEngine:
@@ -137,6 +138,25 @@ class GBEngine:
         c = ctypes.c_int64()
         N.check(N.lib().prx_integrator_step_count(self._h, int(system), ctypes.byref(c)), "prx_integrator_step_count")
         return c.value
+
+    PASSES = ("born_pairs", "born_gather", "force_pairs", "force_gather", "chain_pairs", "bonded", "chain_gather")
+
+    def profile_passes(self, n_reps: int = 10) -> dict:
+        """Mean HIP-event duration (ms) of each force pass launched on its own."""
+        ms = (ctypes.c_double * 7)()
+        N.check(N.lib().prx_profile_passes(self._h, int(n_reps), ms), "prx_profile_passes")
+        return dict(zip(self.PASSES, list(ms)))
+
+    def work_items(self) -> int:
+        n = ctypes.c_int64()
+        N.check(N.lib().prx_engine_work_items(self._h, ctypes.byref(n)), "prx_engine_work_items")
+        return n.value
+
+    def warnings(self, system: int = 0):
+        """Safety-cap event counts (velocity, force, constraint, displacement)."""
+        c = (ctypes.c_int64 * 4)()
+        N.check(N.lib().prx_integrator_warnings(self._h, int(system), c), "prx_integrator_warnings")
+        return list(c)
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:

@@ -72,6 +72,19 @@ class SyntheticProtein:
     def n_atoms(self) -> int:
         return int(self.charges.shape[0])
 
+    def system_dict(self, exclusions: bool = True) -> dict:
+        """PhysicsSystem-style dict; with exclusions, the (N, 32) rows of the
+        bond graph's 1-2/1-3 (scale 0) and 1-4 (0.5 / 0.8333) pairs."""
+        d = self.as_dict()
+        if exclusions:
+            if "excl" not in self.meta:
+                from prolix_amd.physics.neighbor_list import ExclusionSpec, map_exclusions_to_dense_padded
+
+                spec = ExclusionSpec.from_bonds(self.bonds, self.n_atoms)
+                self.meta["excl"] = map_exclusions_to_dense_padded(spec)
+            d["excl_indices"], d["excl_scales_vdw"], d["excl_scales_elec"] = self.meta["excl"]
+        return d
+
     def as_dict(self) -> dict:
         return {
             "positions": self.positions,
@@ -491,12 +504,39 @@ def batch_mix_sizes(n_systems: int = 256, lo: int = 300, hi: int = 5000, seed: i
     return np.exp(rng.uniform(math.log(lo), math.log(hi), size=n_systems)).astype(np.int64)
 
 
+def _mix_member(args):
+    n, sd = args
+    p = make_protein(int(n), sd, radius=compact_radius(int(n)))
+    p.system_dict(exclusions=True)  # build the exclusion rows in the worker
+    return p
+
+
 def batch_mix(n_systems: int = 256, lo: int = 300, hi: int = 5000, seed: int = 2,
-              temperatures=(280.0, 350.0)):
-    """The uneven many-system mix: proteins plus per-system temperatures."""
+              temperatures=(280.0, 350.0), workers: int | None = None):
+    """The uneven many-system mix: proteins plus per-system temperatures.
+
+    Systems are generated in a forked process pool (``workers`` processes, or
+    the CPUs this process may use, at most 16); call it before the process
+    touches the GPU. Each system depends only on its own seed, so the result is
+    the same for any worker count.
+    """
+    import os
+
     sizes = batch_mix_sizes(n_systems, lo, hi, seed)
     rng = np.random.default_rng(seed + 7919)
     temps = rng.uniform(temperatures[0], temperatures[1], size=n_systems)
-    systems = [make_protein(int(n), seed * 100003 + k, radius=compact_radius(int(n)))
-               for k, n in enumerate(sizes)]
+    jobs = [(int(n), seed * 100003 + k) for k, n in enumerate(sizes)]
+    if workers is None:
+        workers = min(16, len(os.sched_getaffinity(0)))
+    if workers <= 1:
+        return [_mix_member(j) for j in jobs], temps
+    import multiprocessing as mp
+
+    # largest first, so the pool's tail is short
+    order = sorted(range(len(jobs)), key=lambda k: -jobs[k][0])
+    with mp.get_context("fork").Pool(workers) as pool:
+        built = pool.map(_mix_member, [jobs[k] for k in order], chunksize=1)
+    systems = [None] * len(jobs)
+    for k, p in zip(order, built):
+        systems[k] = p
     return systems, temps

@@ -54,14 +54,30 @@ def test_zero_friction_matches_float64_verlet():
     eng.close()
     dt = dt_fs / AKMA_FS
     ff = lambda x: O.forces(sysd, x)
-    x, p, f = x0.copy(), p0.copy(), ff(x0)
-    for _ in range(steps):
-        x, p, f = O.baoab_step(x, p, f, m, dt, 0.0, 0.0, np.zeros_like(x), ff)
+
+    def verlet(round32):
+        r32 = (lambda a: a.astype(np.float32).astype(np.float64)) if round32 else (lambda a: a)
+        x, p, f = x0.copy(), p0.copy(), ff(x0)
+        for _ in range(steps):
+            p = r32(p + 0.5 * dt * f)
+            x = r32(x + 0.5 * dt * p / m[:, None])
+            x = r32(x + 0.5 * dt * p / m[:, None])
+            f = r32(ff(x))
+            p = r32(p + 0.5 * dt * f)
+        return x, p
+
+    x, p = verlet(False)
+    x32, p32 = verlet(True)  # the same integration with float32 state
     dx = np.sqrt(np.mean((xg - x) ** 2))
     dp = np.sqrt(np.mean((pg - p) ** 2) / np.mean(p**2))
-    print(f"[zero-friction] rms dx {dx:.2e} A, rel dp {dp:.2e}", flush=True)
-    assert dx < 1e-4
-    assert dp < 1e-3
+    dx32 = np.sqrt(np.mean((x32 - x) ** 2))
+    dp32 = np.sqrt(np.mean((p32 - p) ** 2) / np.mean(p**2))
+    print(f"[zero-friction] device: rms dx {dx:.2e} A, rel dp {dp:.2e}; "
+          f"float32-state oracle: rms dx {dx32:.2e} A, rel dp {dp32:.2e}", flush=True)
+    # Positions and momenta are float32 state (as in prolix's default dtype):
+    # the device must track float64 Verlet as closely as float32 storage allows.
+    assert dx < 4.0 * dx32 + 1e-6
+    assert dp < 4.0 * dp32 + 1e-6
 
 
 def test_nve_drift_three_particles():
