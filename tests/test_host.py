@@ -1,0 +1,174 @@
+"""CPU-side checks: the C-ABI library loads and exports every declared entry
+point, host planning (CMAP coefficients, tiles, validation errors), the
+exclusion builder against graph distances, the seeded generator, and the
+C++/OpenMP restatement against the NumPy oracle."""
+
+from __future__ import annotations
+
+import ctypes
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle import gb_oracle as O
+from prolix_amd import _native as N
+from prolix_amd import synth
+from prolix_amd.physics.neighbor_list import ExclusionSpec, map_exclusions_to_dense_padded, max_exclusion_slots_needed
+from prolix_amd.utils.topology import find_bonded_exclusions
+from tests.helpers import protein
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def declared_symbols():
+    text = (ROOT / "include" / "prolix_gb.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(prx_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = N.lib()
+    names = declared_symbols()
+    assert len(names) >= 20
+    for name in names:
+        assert hasattr(lib, name), name
+    assert lib.prx_abi_version() == 1
+
+
+def test_ctypes_struct_layouts_match_header():
+    """Field counts of the ctypes mirrors equal the C structs' members."""
+    text = (ROOT / "include" / "prolix_gb.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+
+    def members(struct):
+        body = re.search(r"typedef struct %s \{(.*?)\}" % struct, text, re.S).group(1)
+        n = 0
+        for decl in body.split(";"):
+            decl = decl.strip()
+            if decl:
+                n += decl.count(",") + 1
+        return n
+
+    assert members("prx_system_desc") == len(N.SystemDesc._fields_)
+    assert members("prx_langevin_params") == len(N.LangevinParams._fields_)
+    assert members("prx_pool_limits") == len(N.PoolLimits._fields_)
+
+
+def test_native_cmap_coefficients_match_oracle():
+    g = np.random.default_rng(0).uniform(-2, 2, size=(24, 24))
+    assert np.max(np.abs(N.cmap_coefficients(g) - O.cmap_coefficients(g[None])[0])) < 1e-11
+
+
+def test_plan_tiles_and_validation_errors():
+    s = protein(138, 0)
+    d = N.Desc(s)
+    n_items, n_excl = N.plan_tiles(d)
+    assert n_items == 6  # 3 tiles of 64 -> 3 * 4 / 2
+    assert 3 <= n_excl <= 6
+    bad = dict(s)
+    bad["bonds"] = np.array(s["bonds"]).copy()
+    bad["bonds"][0, 1] = 10_000
+    with pytest.raises(N.NativeError, match="bond index out of range"):
+        N.plan_tiles(N.Desc(bad))
+    with pytest.raises(ValueError, match="empty"):
+        N.Desc({"charges": np.zeros(0)})
+
+
+def test_more_than_three_exclusion_classes_rejected():
+    s = protein(40, 1)
+    sv = np.array(s["excl_scales_vdw"], np.float64)
+    rows = np.argwhere(np.array(s["excl_indices"]) >= 0)
+    for k, (i, c) in enumerate(rows[:8]):
+        sv[i, c] = 0.1 * (k + 1)
+    s["excl_scales_vdw"] = sv
+    with pytest.raises(N.NativeError, match="exclusion scale pairs"):
+        N.plan_tiles(N.Desc(s))
+
+
+def _graph_distance_pairs(bonds, n, dist):
+    A = np.zeros((n, n), dtype=np.int64)
+    A[bonds[:, 0], bonds[:, 1]] = 1
+    A[bonds[:, 1], bonds[:, 0]] = 1
+    D = np.full((n, n), 99)
+    np.fill_diagonal(D, 0)
+    reach = np.eye(n, dtype=np.int64)
+    for k in range(1, 4):
+        reach = ((reach @ A) > 0).astype(np.int64)
+        D = np.where((D == 99) & (reach > 0), k, D)
+    iu, ju = np.triu_indices(n, 1)
+    return {(int(i), int(j)) for i, j in zip(iu, ju) if D[i, j] == dist}
+
+
+def test_find_bonded_exclusions_matches_graph_distances():
+    p = synth.make_protein(80, 3)
+    ex = find_bonded_exclusions(p.bonds, p.n_atoms)
+    for k, got in ((1, ex.idx_12), (2, ex.idx_13), (3, ex.idx_14)):
+        assert {tuple(x) for x in got.tolist()} == _graph_distance_pairs(p.bonds, p.n_atoms, k)
+
+
+def test_exclusion_rows_semantics():
+    p = synth.make_protein(60, 4)
+    spec = ExclusionSpec.from_bonds(p.bonds, p.n_atoms)
+    idx, sv, se = map_exclusions_to_dense_padded(spec)
+    assert idx.shape == (60, 32)
+    for i in range(60):
+        row = idx[i][idx[i] >= 0]
+        assert np.all(np.diff(row) > 0)  # sorted, unique (neighbor_list.py:293-296)
+    pairs14 = {tuple(x) for x in spec.idx_14.tolist()}
+    for i, j in list(pairs14)[:20]:
+        k = list(idx[i]).index(j)
+        assert sv[i, k] == np.float32(0.5) and se[i, k] == np.float32(0.83333333)
+    assert max_exclusion_slots_needed(spec) <= 32
+    with pytest.raises(ValueError, match="exclusion slots"):
+        map_exclusions_to_dense_padded(spec, max_exclusions=4)
+
+
+def test_generator_is_seeded_and_sized():
+    a = synth.make_protein(500, 11, radius=synth.compact_radius(500))
+    b = synth.make_protein(500, 11, radius=synth.compact_radius(500))
+    assert a.n_atoms == 500
+    assert np.array_equal(a.positions, b.positions) and np.array_equal(a.bonds, b.bonds)
+    assert abs(a.charges.sum()) < 1e-9
+    assert a.sigmas.min() >= 1.0 and a.sigmas.max() <= 3.5
+    assert np.all(a.scaled_radii < a.radii)
+    bl = a.bond_params[:, 0]
+    assert bl.min() >= 0.99 and bl.max() <= 1.53
+    p = synth.peptide_138(0)
+    assert p.n_atoms == 138 and p.n_residues == 10 and len(p.cmap_torsions) == 8
+    sizes = synth.batch_mix_sizes(256)
+    assert sizes.min() >= 300 and sizes.max() <= 5000 and len(sizes) == 256
+
+
+def test_cpu_restatement_matches_numpy_oracle():
+    from oracle import cpu as C
+
+    for n, seed, cut in ((138, 0, 0.0), (200, 5, 9.0)):
+        s = protein(n, seed, radius=synth.compact_radius(n))
+        cs = C.CpuSystem(s, cutoff=cut)
+        e, f = cs.energy_forces(s["positions"])
+        terms = O.energy_terms(s, s["positions"], cutoff=cut)
+        fo = O.forces(s, s["positions"], cutoff=cut)
+        assert abs(e[-1] - terms["total"]) <= 1e-12 * sum(abs(terms[k]) for k in O.TERM_NAMES)
+        assert np.sqrt(np.mean((f - fo) ** 2) / np.mean(fo**2)) < 1e-12
+        for k, name in enumerate(N.TERMS[:-1]):
+            assert e[k] == pytest.approx(terms[name], rel=1e-10, abs=1e-9), name
+
+
+def test_cpu_restatement_zero_friction_is_velocity_verlet():
+    from oracle import cpu as C
+
+    s = protein(40, 2)
+    cs = C.CpuSystem(s)
+    m = np.asarray(s["masses"])
+    x = np.ascontiguousarray(s["positions"].copy())
+    p = np.ascontiguousarray(np.random.default_rng(0).normal(size=x.shape) * np.sqrt(m[:, None] * 0.6))
+    _, f = cs.energy_forces(x)
+    x0, p0, f0 = x.copy(), p.copy(), f.copy()
+    cs.baoab(x, p, f, 5, 0.5, 300.0, 0.0, 1)
+    dt = 0.5 / 48.88821291839
+    ff = lambda y: O.forces(s, y)
+    for _ in range(5):
+        x0, p0, f0 = O.baoab_step(x0, p0, f0, m, dt, 0.0, 0.0, np.zeros_like(x0), ff)
+    assert np.max(np.abs(x - x0)) < 1e-10

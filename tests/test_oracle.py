@@ -1,0 +1,230 @@
+"""Pins for the float64 oracle: closed forms from prolix's own tests, finite
+differences, invariants, CMAP interpolation properties, Philox known answers."""
+
+from __future__ import annotations
+
+import math
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle import gb_oracle as O
+from oracle import philox
+from tests.helpers import protein
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+# ---------------------------------------------------------------- closed forms
+
+def test_born_ion():
+    """tests/physics/test_generalized_born.py:11-52: B = R - 0.09, E = -1/2 k tau q^2 / B."""
+    B = O.born_radii(np.zeros((1, 3)), np.array([2.0]))
+    assert B[0] == pytest.approx(1.91, rel=1e-14)
+    e = O.gb_energy(np.zeros((1, 3)), np.array([1.0]), B, eps_in=1.0, eps_out=78.5)
+    assert e == pytest.approx(-0.5 * 332.0637 * (1 - 1 / 78.5) / 1.91, rel=1e-9)
+
+
+def test_bond_energy_and_force():
+    """tests/physics/test_bonded.py:10-38: k = 100, dr = 0.1 -> E = 0.5, F = -/+10."""
+    s = {"charges": np.zeros(2), "sigmas": np.ones(2), "epsilons": np.zeros(2), "radii": np.ones(2) * 1.5,
+         "bonds": np.array([[0, 1]]), "bond_params": np.array([[1.0, 100.0]])}
+    pos = np.array([[0.0, 0, 0], [1.1, 0, 0]])
+    t = O.energy_terms(s, pos, implicit_solvent=False)
+    assert t["bond"] == pytest.approx(0.5, rel=1e-12)
+    f = O.forces(s, pos, implicit_solvent=False)
+    assert f[1, 0] == pytest.approx(-10.0, abs=1e-9)
+    assert f[0, 0] == pytest.approx(10.0, abs=1e-9)
+
+
+def test_angle_energy():
+    """tests/physics/test_bonded.py:41-59: 90 degrees vs theta0 = pi, k = 100."""
+    pos = np.array([[1.0, 0, 0], [0.0, 0, 0], [0.0, 1, 0]])
+    e = O.angle_energy(pos, np.array([[0, 1, 2]]), np.array([[math.pi, 100.0]]))
+    assert e == pytest.approx(0.5 * 100 * (math.pi / 2) ** 2, rel=1e-4)
+
+
+def test_lj_pair_value():
+    """tests/physics/test_numerical_stability.py:9-26: r = 1.5, sigma = eps = 1 -> -0.3203."""
+    pos = np.array([[0.0, 0, 0], [1.5, 0, 0]])
+    S = np.ones((2, 2))
+    e = O.lj_energy(pos, np.ones(2), np.ones(2), S, cutoff=2.0)
+    assert abs(e + 0.3203) < 1e-4
+    # overlapping atoms are masked, not NaN (same test, :28-36)
+    e0 = O.lj_energy(np.zeros((2, 3)), np.ones(2), np.ones(2), S, cutoff=2.0)
+    assert e0 == 0.0
+
+
+def test_coulomb_pair_and_exclusion_scale():
+    pos = np.array([[0.0, 0, 0], [2.0, 0, 0]])
+    q = np.array([0.5, -0.4])
+    assert O.coulomb_energy(pos, q, np.ones((2, 2))) == pytest.approx(332.0637 * -0.2 / 2.0, rel=1e-14)
+    idx = np.array([[1, -1], [0, -1]])
+    sc = np.array([[0.8333, 1.0], [0.8333, 1.0]])
+    S = O.dense_scales(idx, sc, 2)
+    assert O.coulomb_energy(pos, q, S) == pytest.approx(0.8333 * 332.0637 * -0.2 / 2.0, rel=1e-14)
+
+
+def test_dihedral_cis_trans():
+    cis = np.array([[1.0, 1, 0], [0.0, 0, 0], [0.0, 0, 1], [1.0, 1, 1]])
+    trans = np.array([[1.0, 0, 0], [0.0, 0, 0], [0.0, 0, 1], [-1.0, 0, 1]])
+    idx = np.array([[0, 1, 2, 3]])
+    assert O.dihedral_angles(cis, idx)[0] == pytest.approx(0.0, abs=1e-7)
+    assert abs(O.dihedral_angles(trans, idx)[0]) == pytest.approx(math.pi, abs=1e-7)
+
+
+def test_pair_integral_limits_and_far_field_identity():
+    # vanishes far away, zero when j's sphere sits inside i's offset sphere
+    assert O.pair_integral(np.array(1e4), 1.5, 1.2) < 1e-12
+    assert O.pair_integral(np.array(0.1), 1.5, 0.2) == 0.0
+    # far field (r - s > rho): I = (x / (1 - x^2) - atanh x) / 2r, x = s / r
+    rng = np.random.default_rng(0)
+    for _ in range(200):
+        s, rho = rng.uniform(0.8, 1.8), rng.uniform(1.1, 1.9)
+        r = s + rho + rng.uniform(0.01, 40)
+        x = s / r
+        assert O.pair_integral(np.array(r), rho, s) == pytest.approx((x / (1 - x * x) - math.atanh(x)) / (2 * r),
+                                                                     rel=1e-9, abs=1e-15)
+
+
+def test_far_field_polynomials_match_series():
+    """The coefficients compiled into the pair kernels reproduce h and k."""
+    txt = (ROOT / "prolix_amd/csrc/prx_obc_farfield.h").read_text()
+    H = [float(v.rstrip("f")) for v in re.search(r"kObcH\[\d+\] = \{(.*?)\}", txt).group(1).split(",")]
+    K = [float(v.rstrip("f")) for v in re.search(r"kObcK\[\d+\] = \{(.*?)\}", txt).group(1).split(",")]
+    y = np.linspace(0.0, 0.36, 2001)
+    h = sum(2 * k / (2 * k + 1) * y ** (k - 1) for k in range(1, 300))
+    kk = sum(2 * k * (k + 1) / (2 * k + 1) * y ** (k - 1) for k in range(1, 300))
+    assert np.max(np.abs(np.polyval(H[::-1], y) / h - 1)) < 2e-7
+    assert np.max(np.abs(np.polyval(K[::-1], y) / kk - 1)) < 2e-7
+    # and the polynomial form equals the oracle's integral and its derivative
+    rng = np.random.default_rng(1)
+    for _ in range(100):
+        s, rho = rng.uniform(0.8, 1.8), rng.uniform(1.1, 1.9)
+        r = max(s + rho, s / 0.6) + rng.uniform(0.01, 30)
+        x = s / r
+        poly_I = 0.5 * x**3 / r * np.polyval(H[::-1], x * x)
+        poly_dI = -(x**3) / r**2 * np.polyval(K[::-1], x * x)
+        assert poly_I == pytest.approx(O.pair_integral(np.array(r), rho, s), rel=3e-7)
+        assert poly_dI == pytest.approx(O.pair_integral_dr(np.array(r), rho, s), rel=3e-7)
+
+
+def test_pair_integral_derivative_fd():
+    rng = np.random.default_rng(2)
+    for _ in range(300):
+        r, rho, s = rng.uniform(0.3, 20), rng.uniform(1.1, 1.9), rng.uniform(0.8, 1.8)
+        if min(abs(abs(r - s) - rho), abs(r + s - rho)) < 1e-3:
+            continue  # kinks of the max / step
+        h = 1e-6
+        fd = (O.pair_integral(np.array(r + h), rho, s) - O.pair_integral(np.array(r - h), rho, s)) / (2 * h)
+        assert O.pair_integral_dr(np.array(r), rho, s) == pytest.approx(fd, rel=1e-6, abs=1e-9)
+
+
+# ---------------------------------------------------------------- forces
+
+def _fd_forces(sysd, pos, h=1e-5, **kw):
+    coeff = O.cmap_coefficients(sysd["cmap_energy_grids"]) if sysd.get("cmap_torsions") is not None else None
+    f = np.zeros_like(pos)
+    for i in range(pos.shape[0]):
+        for k in range(3):
+            xp = pos.copy(); xp[i, k] += h
+            xm = pos.copy(); xm[i, k] -= h
+            f[i, k] = -(O.energy_terms(sysd, xp, cmap_coeffs=coeff, **kw)["total"]
+                        - O.energy_terms(sysd, xm, cmap_coeffs=coeff, **kw)["total"]) / (2 * h)
+    return f
+
+
+@pytest.mark.parametrize("cutoff", [0.0, 6.0])
+def test_forces_match_finite_differences(cutoff):
+    s = protein(30, 7)
+    f = O.forces(s, s["positions"], cutoff=cutoff)
+    fd = _fd_forces(s, s["positions"], cutoff=cutoff)
+    assert np.sqrt(np.mean((f - fd) ** 2) / np.mean(fd**2)) < 1e-7
+
+
+def test_harmonic_improper_and_exception_forces_fd():
+    s = protein(24, 8)
+    rng = np.random.default_rng(3)
+    n_imp = len(s["impropers"])
+    s["improper_params"] = np.stack([rng.uniform(1, 5, n_imp), rng.uniform(-0.5, 0.5, n_imp)], 1)[:, None, :]
+    s["exception_pairs"] = np.array([[0, 10], [2, 20]])
+    s["exception_sigmas"] = np.array([2.5, 3.0])
+    s["exception_epsilons"] = np.array([0.1, 0.2])
+    s["exception_chargeprods"] = np.array([0.05, -0.1])
+    s["urey_bradley_bonds"] = s["angles"][:5, [0, 2]]
+    s["urey_bradley_params"] = np.stack([np.full(5, 2.5), np.full(5, 20.0)], 1)
+    f = O.forces(s, s["positions"])
+    fd = _fd_forces(s, s["positions"])
+    assert np.sqrt(np.mean((f - fd) ** 2) / np.mean(fd**2)) < 1e-7
+
+
+def test_newton_and_rigid_motion_invariance():
+    s = protein(60, 9)
+    pos = s["positions"]
+    f = O.forces(s, pos)
+    assert np.abs(f.sum(0)).max() < 1e-9 * np.abs(f).max()
+    e0 = O.total_energy(s, pos)
+    th = 0.7
+    R = np.array([[math.cos(th), -math.sin(th), 0], [math.sin(th), math.cos(th), 0], [0, 0, 1]])
+    e1 = O.total_energy(s, pos @ R.T + np.array([3.0, -2.0, 5.0]))
+    assert e1 == pytest.approx(e0, rel=1e-10)
+
+
+# ---------------------------------------------------------------- CMAP
+
+def test_bicubic_patch_reproduces_corner_data():
+    W = O._bicubic_inverse()
+    rng = np.random.default_rng(4)
+    rhs = rng.normal(size=16)
+    c = (W @ rhs).reshape(4, 4)
+    P = lambda a, b: sum(c[p, q] * a**p * b**q for p in range(4) for q in range(4))
+    Pa = lambda a, b: sum(p * c[p, q] * a ** max(p - 1, 0) * b**q for p in range(1, 4) for q in range(4))
+    for ci, (a, b) in enumerate([(0, 0), (1, 0), (1, 1), (0, 1)]):
+        assert P(a, b) == pytest.approx(rhs[ci], abs=1e-12)
+        assert Pa(a, b) == pytest.approx(rhs[4 + ci], abs=1e-12)
+
+
+def test_cmap_interpolant_hits_nodes_and_tracks_smooth_map():
+    G = 24
+    x = 2 * np.pi * np.arange(G) / G
+    phi, psi = np.meshgrid(x, x, indexing="ij")
+    Ef = np.sin(phi) * np.cos(2 * psi) + 0.3 * np.cos(phi + psi)
+    grid = Ef.T  # stored [psi, phi]
+    coeff = O.cmap_coefficients(grid[None])
+    # nodes
+    e = O.cmap_energy(phi.ravel(), psi.ravel(), np.zeros(G * G, int), coeff)
+    assert np.allclose(e, Ef.ravel(), atol=1e-12)
+    # between nodes, a smooth periodic map is reproduced closely, values and slopes
+    rng = np.random.default_rng(5)
+    a, b = rng.uniform(-np.pi, np.pi, 500), rng.uniform(-np.pi, np.pi, 500)
+    e, dphi, dpsi = O.cmap_energy(a, b, np.zeros(500, int), coeff, return_grad=True)
+    exact = np.sin(a) * np.cos(2 * b) + 0.3 * np.cos(a + b)
+    assert np.max(np.abs(e - exact)) < 2e-3
+    assert np.max(np.abs(dphi - (np.cos(a) * np.cos(2 * b) - 0.3 * np.sin(a + b)))) < 2e-2
+
+
+def test_spline_derivatives_of_sinusoid():
+    G = 24
+    x = np.linspace(0, 2 * np.pi, G + 1)
+    y = np.sin(x)
+    d = O.spline_derivative_at_nodes(x, y, O.periodic_spline_second_derivs(x, y))
+    assert np.max(np.abs(d - np.cos(x[:-1]))) < 1e-4
+
+
+# ---------------------------------------------------------------- Philox
+
+def test_philox_known_answers():
+    """Known-answer vectors published with Random123 for philox4x32-10."""
+    assert philox.philox4x32_10((0, 0, 0, 0), (0, 0)) == (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)
+    assert philox.philox4x32_10((0xFFFFFFFF,) * 4, (0xFFFFFFFF, 0xFFFFFFFF)) == (
+        0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)
+    assert philox.philox4x32_10((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0)) == (
+        0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1)
+
+
+def test_box_muller_stream_statistics():
+    xs = np.array([philox.normal3(12345, s, i) for s in range(40) for i in range(100)]).ravel()
+    assert abs(xs.mean()) < 0.05
+    assert abs(xs.std() - 1.0) < 0.05
