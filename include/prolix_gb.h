@@ -161,6 +161,9 @@ int prx_integrator_init(prx_engine* eng, const float* positions, const float* mo
 /* n_steps BAOAB steps of every system (settle.py:1699-1744, jax_md
  * nvt_langevin apply_fn), replayed from a captured hipGraph. */
 int prx_integrator_steps(prx_engine* eng, int32_t n_steps, void* stream);
+/* Replace the atoms' masses (host, packed, amu) before prx_integrator_init;
+ * nvt_langevin's `mass` argument (settle.py:1676-1697). */
+int prx_engine_set_masses(prx_engine* eng, const double* masses_amu);
 /* Copy state out (device pointers, packed; any may be NULL). */
 int prx_integrator_state(prx_engine* eng, float* positions, float* momenta, float* forces,
                          void* stream);

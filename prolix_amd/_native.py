@@ -73,6 +73,7 @@ _SIGS = {
     "prx_integrator_init": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                            ctypes.POINTER(LangevinParams), ctypes.c_void_p]),
     "prx_integrator_steps": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]),
+    "prx_engine_set_masses": (ctypes.c_int, [ctypes.c_void_p, c_f64p]),
     "prx_integrator_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                             ctypes.c_void_p]),
     "prx_integrator_kinetic": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),

@@ -212,23 +212,20 @@ __global__ __launch_bounds__(256) void k_chain_gather(DeviceView v, int mode) {
   const float3 fb = bonded_force(v, c.g);
   float3 f = make_float3((float)(a.x + (double)fp.x) + fb.x, (float)(a.y + (double)fp.y) + fb.y,
                          (float)(a.z + (double)fp.z) + fb.z);
-  if (mode < 0) {
-    v.force[c.g] = make_float4(f.x, f.y, f.z, 0.0f);
-    return;
-  }
   const SlotConst& S = v.slots[c.slot];
-  if (!S.active) return;
   if (S.caps) {
-    // non-finite -> 0, then |F| <= kForceCap (batched_simulate.py:323-330)
+    // non-finite -> 0, then |F| <= kForceCap (batched_simulate.py:323-330). Also in
+    // mode -1, so a force recomputed at unchanged positions is the one the step stored.
     if (!isfinite(f.x) || !isfinite(f.y) || !isfinite(f.z)) f = make_float3(0.f, 0.f, 0.f);
     const float fm = sqrtf(f.x * f.x + f.y * f.y + f.z * f.z + 1e-12f);
     if (fm > kForceCap) {
       const float sc = kForceCap / fm;
       f = make_float3(f.x * sc, f.y * sc, f.z * sc);
-      atomicAdd(&v.warn[c.slot * 4 + kWarnForceCap], 1u);
+      if (mode >= 0) atomicAdd(&v.warn[c.slot * 4 + kWarnForceCap], 1u);
     }
   }
   v.force[c.g] = make_float4(f.x, f.y, f.z, 0.0f);
+  if (mode < 0 || !S.active) return;
   float4 p = v.mom[c.g];  // w = 1/m
   p.x = fmaf(S.half_dt, f.x, p.x);
   p.y = fmaf(S.half_dt, f.y, p.y);

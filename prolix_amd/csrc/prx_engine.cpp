@@ -880,6 +880,29 @@ int prx_integrator_steps(prx_engine* eng, int32_t n_steps, void* stream) {
   });
 }
 
+int prx_engine_set_masses(prx_engine* eng, const double* masses) {
+  return guard([&] {
+    need(eng && masses, "prx_engine_set_masses: bad arguments");
+    eng->rebuild();
+    HIPCHECK(hipDeviceSynchronize());
+    size_t p = 0;
+    for (int32_t sid : eng->order) {
+      const Slot& s = eng->slots[sid];
+      std::vector<float4> mom(s.n);
+      std::vector<float> sqm(s.n);
+      HIPCHECK(hipMemcpy(mom.data(), eng->v.mom + s.a0, s.n * sizeof(float4), hipMemcpyDeviceToHost));
+      for (int i = 0; i < s.n; ++i, ++p) {
+        const double m = masses[p];
+        need(m > 0.0, "prx_engine_set_masses: masses must be positive");
+        mom[i].w = (float)(1.0 / m);
+        sqm[i] = (float)std::sqrt(m);
+      }
+      HIPCHECK(hipMemcpy(eng->v.mom + s.a0, mom.data(), s.n * sizeof(float4), hipMemcpyHostToDevice));
+      HIPCHECK(hipMemcpy(eng->v.sqrt_m + s.a0, sqm.data(), s.n * sizeof(float), hipMemcpyHostToDevice));
+    }
+  });
+}
+
 int prx_integrator_state(prx_engine* eng, float* positions, float* momenta, float* forces, void* stream) {
   return guard([&] {
     need(eng != nullptr, "prx_integrator_state: bad arguments");

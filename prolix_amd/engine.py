@@ -117,6 +117,13 @@ class GBEngine:
         arr = (N.LangevinParams * self.n_systems)(*[p.c() for p in params])
         N.check(N.lib().prx_integrator_init(self._h, _vp(x), _vp(m), arr, _stream(t)), "prx_integrator_init")
 
+    def set_masses(self, masses_amu):
+        """Per-atom masses (packed, amu) used by the integrator from the next init."""
+        m = np.ascontiguousarray(np.asarray(masses_amu, np.float64).reshape(-1))
+        if m.shape[0] != self.n_atoms:
+            raise ValueError("one mass per atom is required")
+        N.check(N.lib().prx_engine_set_masses(self._h, m.ctypes.data_as(N.c_f64p)), "prx_engine_set_masses")
+
     def steps(self, n_steps: int):
         N.check(N.lib().prx_integrator_steps(self._h, int(n_steps), _stream(self.torch)), "prx_integrator_steps")
 
@@ -157,6 +164,81 @@ class GBEngine:
         c = (ctypes.c_int64 * 4)()
         N.check(N.lib().prx_integrator_warnings(self._h, int(system), c), "prx_integrator_warnings")
         return list(c)
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self.torch.cuda.synchronize()
+            N.check(N.lib().prx_engine_destroy(self._h), "prx_engine_destroy")
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class GBPool:
+    """A serving pool: systems are admitted and released while others keep stepping."""
+
+    def __init__(self, max_atoms: int, max_systems: int = 1024, max_bonded: int | None = None,
+                 max_cmap_coeff: int | None = None, max_excl_tiles: int | None = None):
+        torch = _torch()
+        self.torch = torch
+        lim = N.PoolLimits(int(max_atoms), int(max_bonded or 12 * max_atoms),
+                           int(max_cmap_coeff or max(1, max_systems) * 4 * 24 * 24 * 16),
+                           int(max_excl_tiles or max(64, max_atoms // 16)), int(max_systems))
+        h = ctypes.c_void_p()
+        N.check(N.lib().prx_pool_create(ctypes.byref(lim), ctypes.byref(h)), "prx_pool_create")
+        self._h = h
+        self._descs = {}
+        self.sizes = {}
+
+    def admit(self, sysd: dict, params: Langevin, positions=None, **opts) -> int:
+        """Upload one system, draw its momenta, compute its forces; returns its slot."""
+        d = N.Desc(sysd, **opts)
+        x = np.ascontiguousarray(np.asarray(sysd["positions"] if positions is None else positions, np.float32))
+        slot = ctypes.c_int32()
+        p = params.c()
+        code = N.lib().prx_pool_admit(self._h, ctypes.byref(d.desc), x.ctypes.data_as(N.c_f32p), ctypes.byref(p),
+                                      ctypes.byref(slot), _stream(self.torch))
+        if code == 3:  # PRX_ERR_CAPACITY
+            return -1
+        N.check(code, "prx_pool_admit")
+        self._descs[slot.value] = d
+        self.sizes[slot.value] = d.n_atoms
+        return slot.value
+
+    def release(self, slot: int):
+        N.check(N.lib().prx_pool_release(self._h, int(slot), _stream(self.torch)), "prx_pool_release")
+        self._descs.pop(slot, None)
+        self.sizes.pop(slot, None)
+
+    def steps(self, n_steps: int):
+        N.check(N.lib().prx_integrator_steps(self._h, int(n_steps), _stream(self.torch)), "prx_integrator_steps")
+
+    def read(self, slot: int):
+        n = self.sizes[slot]
+        x = np.zeros((n, 3), np.float32)
+        p = np.zeros((n, 3), np.float32)
+        f = np.zeros((n, 3), np.float32)
+        c = ctypes.c_int64()
+        N.check(N.lib().prx_pool_read(self._h, int(slot), x.ctypes.data_as(N.c_f32p), p.ctypes.data_as(N.c_f32p),
+                                      f.ctypes.data_as(N.c_f32p), ctypes.byref(c), _stream(self.torch)),
+                "prx_pool_read")
+        return x, p, f, c.value
+
+    def energies(self, slot: int):
+        e = np.zeros(N.N_ENERGY)
+        N.check(N.lib().prx_pool_energy(self._h, int(slot), e.ctypes.data_as(N.c_f64p), _stream(self.torch)),
+                "prx_pool_energy")
+        return e
+
+    def live(self):
+        n = ctypes.c_int32()
+        buf = (ctypes.c_int32 * 65536)()
+        N.check(N.lib().prx_pool_slots(self._h, buf, 65536, ctypes.byref(n)), "prx_pool_slots")
+        return list(buf[: n.value])
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:

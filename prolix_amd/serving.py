@@ -1,0 +1,160 @@
+"""Continuous-batching request loop over a GBPool.
+
+Requests (a system, a step count, a temperature, a Philox seed) arrive over
+time. The loop admits every arrived request whose estimated device footprint
+fits the HBM budget (and the pool's arenas), advances all live systems
+together in chunks of steps (one captured graph per chunk), and as soon as a
+system has taken its steps reads its state back and evicts it, so waiting
+requests take its place while the others keep running. Positions, momenta,
+forces, Born radii and Philox counters stay on the device for a system's whole
+life. Reported: per-request completion latency (finish - arrival), p50/p99,
+aggregate ns/day and pair interactions/s.
+
+prolix has no serving loop; its batched path pads systems into buckets and
+runs them to completion together (padding.py:573-654,
+batched_simulate.py:1125-1422).
+"""
+
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from prolix_amd.engine import GBPool, Langevin
+
+TILE = 64
+
+
+def hbm_bytes(n_atoms: int, n_bonded: int = 0) -> int:
+    """Device footprint of one system: per-atom arrays (~330 B), bonded entries
+    with force records and energy slots (~120 B each) and the pair work list
+    with its 2 x 64 float4 partial slots and energies (~2.1 KB per 64x64 tile pair)."""
+    nt = (n_atoms + TILE - 1) // TILE
+    return 330 * n_atoms + 120 * n_bonded + 2100 * nt * (nt + 1) // 2
+
+
+@dataclass
+class Request:
+    system: dict
+    n_steps: int
+    temperature: float
+    seed: int
+    arrival: float
+    dt_fs: float = 1.0
+    gamma_ps: float = 1.0
+    rid: int = 0
+
+
+@dataclass
+class Result:
+    rid: int
+    n_atoms: int
+    n_steps: int
+    arrival: float
+    admitted: float
+    finished: float
+    positions: np.ndarray
+    momenta: np.ndarray
+    forces: np.ndarray
+    energies: np.ndarray | None = None
+    extra: dict = field(default_factory=dict)
+
+    @property
+    def latency(self) -> float:
+        return self.finished - self.arrival
+
+
+def _bonded(sysd) -> int:
+    n = 0
+    for k in ("bonds", "angles", "dihedrals", "impropers", "cmap_torsions", "urey_bradley_bonds"):
+        v = sysd.get(k)
+        if v is not None:
+            n += len(v) * (2 if k == "dihedrals" else 1)
+    return n
+
+
+def serve(requests, hbm_budget_bytes: float = 200e9, max_systems: int = 1024, chunk_steps: int = 100,
+          max_atoms: int | None = None, sample_energies=(), safety_caps: bool = True, clock=None, log=None):
+    """Run the request trace to completion; returns (results, stats)."""
+    import torch
+
+    reqs = sorted(requests, key=lambda r: r.arrival)
+    if max_atoms is None:
+        max_atoms = max(1, int(min(sum(len(r.system["charges"]) for r in reqs), 8_000_000)))
+    pool = GBPool(max_atoms=max_atoms, max_systems=max_systems)
+    clock = clock or time.perf_counter
+    t0 = clock()
+    now = lambda: clock() - t0
+    waiting = list(reqs)
+    live = {}  # slot -> [request, remaining, admitted time, footprint]
+    results = []
+    used = 0
+    pair_steps = 0.0
+    sys_steps = 0
+    sample = set(sample_energies)
+    while waiting or live:
+        t = now()
+        # admit arrived requests in arrival order while the budget allows
+        while waiting and waiting[0].arrival <= t and len(live) < max_systems:
+            r = waiting[0]
+            cost = hbm_bytes(len(r.system["charges"]), _bonded(r.system))
+            if used + cost > hbm_budget_bytes and live:
+                break
+            slot = pool.admit(r.system, Langevin(r.dt_fs, r.temperature, r.gamma_ps, r.seed, safety_caps))
+            if slot < 0:
+                if not live:
+                    raise RuntimeError(f"request {r.rid} ({len(r.system['charges'])} atoms) does not fit the pool")
+                break
+            waiting.pop(0)
+            used += cost
+            live[slot] = [r, r.n_steps, now(), cost]
+        if not live:
+            wait = waiting[0].arrival - now()
+            if wait > 0:
+                time.sleep(min(wait, 0.05))
+            continue
+        chunk = min(chunk_steps, min(v[1] for v in live.values()))
+        pool.steps(chunk)
+        torch.cuda.synchronize()
+        for v in live.values():
+            v[1] -= chunk
+            n = len(v[0].system["charges"])
+            pair_steps += chunk * n * (n - 1) / 2.0
+            sys_steps += chunk
+        for slot in [s for s, v in live.items() if v[1] <= 0]:
+            r, _, adm, cost = live.pop(slot)
+            x, p, f, steps = pool.read(slot)
+            e = pool.energies(slot) if r.rid in sample else None
+            results.append(Result(r.rid, len(r.system["charges"]), steps, r.arrival, adm, now(), x, p, f, e))
+            pool.release(slot)
+            used -= cost
+            if log:
+                log(f"request {r.rid}: {len(r.system['charges'])} atoms, {steps} steps, "
+                    f"latency {results[-1].latency:.2f}s, live {len(live)}, waiting {len(waiting)}")
+    wall = now()
+    pool.close()
+    lat = np.array([r.latency for r in results])
+    stats = {
+        "requests": len(results),
+        "wall_s": wall,
+        "latency_p50_s": float(np.percentile(lat, 50)),
+        "latency_p99_s": float(np.percentile(lat, 99)),
+        "aggregate_ns_per_day": sys_steps * 1e-6 * 86400.0 / wall * (reqs[0].dt_fs if reqs else 1.0),
+        "pair_interactions_per_s": pair_steps / wall,
+        "offered_pair_steps": float(sum(len(r.system["charges"]) ** 2 / 2.0 * r.n_steps for r in reqs)),
+    }
+    return sorted(results, key=lambda r: r.rid), stats
+
+
+def poisson_trace(systems, rate_per_s: float, steps_range, temps_range=(280.0, 350.0), seed: int = 4,
+                  dt_fs: float = 1.0, gamma_ps: float = 1.0):
+    """Seeded finite trace: Poisson arrivals, uniform step counts and temperatures, one seed each."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate_per_s, size=len(systems)))
+    t -= t[0]
+    steps = rng.integers(steps_range[0], steps_range[1] + 1, size=len(systems))
+    temps = rng.uniform(temps_range[0], temps_range[1], size=len(systems))
+    return [Request(s, int(steps[k]), float(temps[k]), int(rng.integers(1, 2**62)), float(t[k]), dt_fs, gamma_ps, k)
+            for k, s in enumerate(systems)]

@@ -1,0 +1,135 @@
+"""Drop-in API, noise stream and serving pool on the device."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import gb_oracle as O
+from oracle import philox
+from prolix_amd import synth
+from tests.helpers import f32_round, protein
+
+pytestmark = pytest.mark.gpu
+
+KB = 0.0019872041
+
+
+def test_make_energy_fn_dropin_matches_oracle():
+    """make_energy_fn(displacement_fn, system, implicit_solvent=True, exclusion_spec=spec, cutoff=0)."""
+    from prolix_amd.physics.neighbor_list import ExclusionSpec
+    from prolix_amd.physics.system import make_energy_fn
+
+    p = synth.peptide_138(0)
+    sysd = f32_round(p.as_dict())
+    sysd.pop("cmap_torsions")  # dict systems carry no CMAP (PhysicsSystem.from_dict)
+    spec = ExclusionSpec.from_bonds(sysd["bonds"], p.n_atoms)
+    print("[gpu] make_energy_fn", flush=True)
+    fn = make_energy_fn(None, sysd, implicit_solvent=True, exclusion_spec=spec, cutoff=0)
+    e = fn(sysd["positions"], soft_core_lambda=1.0)  # unknown kwargs ignored (system.py:229)
+    ref = dict(sysd)
+    ref["excl_indices"], ref["excl_scales_vdw"], ref["excl_scales_elec"] = \
+        __import__("prolix_amd.physics.neighbor_list", fromlist=["x"]).map_exclusions_to_dense_padded(spec)
+    terms = O.energy_terms(ref, sysd["positions"])
+    assert e == pytest.approx(terms["total"], rel=1e-6)
+    f = fn.forces(sysd["positions"]).cpu().numpy()
+    fo = O.forces(ref, sysd["positions"])
+    assert np.sqrt(np.mean((f - fo) ** 2) / np.mean(fo**2)) < 1e-5
+    d = make_energy_fn(None, sysd, implicit_solvent=True, exclusion_spec=spec, cutoff=0, return_decomposed=True)
+    e_gb, e_direct, born = d["electrostatics"](sysd["positions"])
+    assert e_gb == pytest.approx(terms["gb"], rel=1e-6)
+    assert e_direct == pytest.approx(terms["coulomb"], rel=1e-6)
+    assert np.allclose(born.cpu().numpy(), terms["_born_radii"], rtol=1e-5)
+    # default cutoff is 9 A for LJ/Coulomb (system.py:134)
+    fn9 = make_energy_fn(None, sysd, implicit_solvent=True, exclusion_spec=spec)
+    assert fn9(sysd["positions"]) == pytest.approx(O.energy_terms(ref, sysd["positions"], cutoff=9.0)["total"],
+                                                   rel=1e-6)
+    with pytest.raises(ValueError):
+        make_energy_fn(None, {"charges": np.zeros(0), "sigmas": np.zeros(0), "epsilons": np.zeros(0)})
+
+
+def test_run_simulation_and_nvt_langevin():
+    from prolix_amd.physics import simulate as S
+    from prolix_amd.physics.neighbor_list import ExclusionSpec
+    from prolix_amd.physics.system import make_energy_fn
+
+    p = synth.peptide_138(0)
+    sysd = p.system_dict()
+    spec = ExclusionSpec.from_bonds(sysd["bonds"], p.n_atoms)
+    print("[gpu] run_simulation", flush=True)
+    r = S.run_simulation(sysd, sysd["positions"], temperature=300.0, min_steps=50, therm_steps=200,
+                         key=np.array([0, 7]), exclusion_spec=spec)
+    assert r.shape == (138, 3) and bool(np.isfinite(r.cpu().numpy()).all())
+    fn = make_energy_fn(None, sysd, implicit_solvent=True, exclusion_spec=spec, cutoff=0)
+    init_fn, apply_fn = S.nvt_langevin(fn, None, dt=1e-3, kT=KB * 300.0, gamma=1.0,
+                                       mass=np.asarray(sysd["masses"]) / 418.4)
+    st = init_fn(np.array([0, 1]), r)
+    st = apply_fn(st, n_steps=20)
+    assert bool(np.isfinite(st.position.cpu().numpy()).all())
+
+
+def test_initial_momenta_follow_philox_stream():
+    """Device draws equal the oracle's Philox4x32-10 + Box-Muller stream."""
+    from prolix_amd.engine import GBEngine, Langevin
+
+    s = protein(200, 3)
+    eng = GBEngine(s)
+    eng.init(s["positions"], Langevin(1.0, 310.0, 1.0, seed=123456789))
+    p = eng.state()[1].cpu().numpy().astype(np.float64)
+    eng.close()
+    ref = philox.initial_momenta(123456789, s["masses"], KB * 310.0)
+    assert np.max(np.abs(p - ref)) < 2e-5 * np.max(np.abs(ref))
+
+
+def _pool_run(systems_in_order, schedule, params):
+    """schedule: list of ('admit', k) | ('steps', n) | ('read', k)."""
+    from prolix_amd.engine import GBPool
+
+    pool = GBPool(max_atoms=sum(len(s["charges"]) for s in systems_in_order) + 10, max_systems=16)
+    slots = {}
+    out = {}
+    for op, arg in schedule:
+        if op == "admit":
+            slots[arg] = pool.admit(systems_in_order[arg], params[arg])
+        elif op == "steps":
+            pool.steps(arg)
+        elif op == "read":
+            out[arg] = pool.read(slots[arg])
+            pool.release(slots[arg])
+    pool.close()
+    return out
+
+
+def test_pool_result_is_the_same_alone_and_in_the_mix():
+    from prolix_amd.engine import Langevin
+
+    systems = [protein(n, 60 + k, radius=synth.compact_radius(n)) for k, n in enumerate((150, 420, 97, 260))]
+    params = [Langevin(1.0, 290.0 + 10 * k, 1.0, seed=900 + k, safety_caps=True) for k in range(4)]
+    print("[gpu] pool alone", flush=True)
+    alone = _pool_run(systems, [("admit", 0), ("steps", 150), ("read", 0)], params)
+    print("[gpu] pool mix", flush=True)
+    mixed = _pool_run(systems, [("admit", 1), ("admit", 2), ("steps", 40), ("admit", 0), ("steps", 30),
+                                ("read", 2), ("admit", 3), ("steps", 120), ("read", 0), ("read", 1)], params)
+    xa, pa, fa, na = alone[0]
+    xm, pm, fm, nm = mixed[0]
+    assert na == nm == 150
+    assert np.array_equal(xa, xm) and np.array_equal(pa, pm) and np.array_equal(fa, fm)
+
+
+def test_serving_loop_small_trace():
+    from prolix_amd.serving import poisson_trace, serve
+
+    systems = [protein(int(n), 80 + k, radius=synth.compact_radius(int(n)))
+               for k, n in enumerate((220, 800, 350, 1500, 260, 640))]
+    reqs = poisson_trace(systems, rate_per_s=50.0, steps_range=(100, 400), seed=3)
+    print("[gpu] serve", flush=True)
+    results, stats = serve(reqs, chunk_steps=50, sample_energies=(0, 3))
+    assert len(results) == len(reqs)
+    for r, q in zip(results, reqs):
+        assert r.n_steps == q.n_steps
+        assert np.isfinite(r.positions).all()
+    assert stats["latency_p99_s"] >= stats["latency_p50_s"] > 0
+    e0 = results[0].energies
+    sysd = dict(reqs[0].system)
+    terms = O.energy_terms(sysd, results[0].positions.astype(np.float64))
+    assert e0[-1] == pytest.approx(terms["total"], rel=1e-5, abs=1e-3)
