@@ -10,7 +10,13 @@ advances every system by one BAOAB step.
 Printed JSON line: aggregate ns/day over all systems (value), pair
 interactions/s, the roofline of the dominant pair kernel (HIP-event timed),
 the plain one-system-per-engine path on the same mix, and a CPU baseline
-(the C++/OpenMP restatement, float32) on a bounded sample.
+(the C++/OpenMP restatement, float32) on a bounded sample. Beside them:
+- "serving": a finite seeded Poisson trace (50 requests/s, N log-uniform in
+  [200, 8000], n_steps uniform in [1000, 50000], T U(280, 350) K, own Philox
+  seed each) replayed through the continuous-batching loop: p50/p99
+  completion latency, aggregate ns/day, offered vs achieved pair-steps/s;
+- "large_cutoff": one 20,000-atom globule with a 20 A cutoff and the
+  neighbour-list GB form, ms/step and interactions/s within the cutoff.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -93,6 +99,9 @@ def main():
     ap.add_argument("--min-steps", type=int, default=200)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=32)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--out", default=None)
     a = ap.parse_args()
     if a.gpus != 1:
@@ -100,11 +109,20 @@ def main():
               "independent systems would shard by whole systems (see DESIGN.md).", file=sys.stderr)
         sys.exit(2)
 
-    # 1. synthetic mix, generated before the process touches the GPU (forked pool)
+    # 1. synthetic inputs, generated before the process touches the GPU (forked pool)
     from prolix_amd import synth
 
     t0 = time.time()
-    prots, temps = synth.batch_mix(a.n_systems, 300, 5000, seed=2)
+    jobs = synth.mix_jobs(a.n_systems, 300, 5000, seed=2)
+    n_mix = len(jobs)
+    if not a.no_serve:
+        jobs += synth.mix_jobs(a.serve_requests, 200, 8000, seed=5)
+    if not a.no_large:
+        jobs.append((20000, 7))
+    built = synth.build_many(jobs)
+    prots, temps = built[:n_mix], synth.mix_temperatures(a.n_systems, seed=2)
+    serve_prots = built[n_mix:n_mix + (0 if a.no_serve else a.serve_requests)]
+    large = None if a.no_large else built[-1]
     systems = [p.system_dict() for p in prots]
     sizes = np.array([p.n_atoms for p in prots], np.int64)
     pairs_per_step = float(np.sum(sizes * (sizes - 1) / 2))
@@ -185,7 +203,54 @@ def main():
         log(f"plain path: {tot * 1e3:.2f} ms per mix step ({time.time() - t_all:.0f}s), "
             f"batched speedup {plain['batched_speedup']:.2f}x")
 
-    # 6. CPU baseline (bounded sample)
+    # 6. one large system with a cutoff (20,000 atoms, 20 A, neighbour-list GB form)
+    large_out = None
+    if large is not None:
+        from scipy.spatial import cKDTree
+
+        ls = large.system_dict()
+        rc = 20.0
+        e2 = GBEngine(ls, cutoff=rc, gb_cutoff=True)
+        xl = minimize(e2, ls["positions"], steps=100)
+        e2.init(xl, Langevin(DT_FS, 300.0, 1.0, seed=77, safety_caps=True))
+        e2.steps(20)
+        torch.cuda.synchronize()
+        n_large = 200
+        t1 = time.perf_counter()
+        e2.steps(n_large)
+        torch.cuda.synchronize()
+        tl = (time.perf_counter() - t1) / n_large
+        xe = e2.state()[0].cpu().numpy()
+        n_in = len(cKDTree(xe).query_pairs(rc))
+        lp = e2.profile_passes(5)
+        e2.close()
+        nl = len(ls["charges"])
+        large_out = {"atoms": nl, "cutoff_A": rc, "gb_cutoff": True, "ms_per_step": tl * 1e3,
+                     "ns_per_day": ns_per_day(1.0 / tl), "pairs_within_cutoff": n_in,
+                     "interactions_per_s": n_in / tl, "all_pairs_equivalent_per_s": nl * (nl - 1) / 2 / tl,
+                     "pass_ms": lp}
+        log(f"large cutoff system: {tl * 1e3:.3f} ms/step, {n_in} pairs within {rc} A")
+
+    # 7. serving replay: Poisson arrivals into the continuous-batching loop
+    serving = None
+    if serve_prots:
+        from prolix_amd.serving import poisson_trace, serve
+
+        reqs = poisson_trace([p.system_dict() for p in serve_prots], rate_per_s=50.0, steps_range=(1000, 50000),
+                             seed=6)
+        t1 = time.time()
+        _, st = serve(reqs, hbm_budget_bytes=200e9, chunk_steps=200)
+        offered_rate = 50.0 * st["offered_pair_steps"] / len(reqs)
+        serving = {"requests": st["requests"], "rate_per_s": 50.0, "latency_p50_s": st["latency_p50_s"],
+                   "latency_p99_s": st["latency_p99_s"], "aggregate_ns_per_day": st["aggregate_ns_per_day"],
+                   "pair_interactions_per_s": st["pair_interactions_per_s"], "wall_s": st["wall_s"],
+                   "offered_pair_steps_per_s": offered_rate,
+                   "trace": f"{len(reqs)} requests, Poisson 50/s, N log-uniform [200, 8000], "
+                            "n_steps U[1000, 50000], T U(280, 350) K, HBM budget 200 GB"}
+        log(f"serving: {len(reqs)} requests in {time.time() - t1:.1f}s, p50 {st['latency_p50_s']:.2f}s "
+            f"p99 {st['latency_p99_s']:.2f}s")
+
+    # 8. CPU baseline (bounded sample)
     cpu = None
     if not a.no_cpu:
         t0 = time.time()
@@ -203,17 +268,21 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 pair math, fixed-point force sums, fp64 reductions",
+        "dtype": "fp32",
+        "precision": "fp32 pair arithmetic; fixed-order fp64 partial-sum gathers; fp64 Born finish and "
+                     "energy / kinetic-energy reductions",
         "data": "synthetic",
         "config": {"workload": "batched mix: 256 synthetic proteins, N log-uniform [300, 5000], no cutoff, "
                                "BAOAB dt 1 fs, gamma 1/ps, T U(280,350) K",
                    "n_systems": len(systems), "atoms": int(sizes.sum()), "pairs_per_step": pairs_per_step},
         "pair_interactions_per_s": pairs_per_step * sps,
-        "latency_p50_s": None,
-        "latency_p99_s": None,
+        "latency_p50_s": serving["latency_p50_s"] if serving else None,
+        "latency_p99_s": serving["latency_p99_s"] if serving else None,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "plain_path": plain,
+        "serving": serving,
+        "large_cutoff": large_out,
         "temperature_K": {"mean": float(np.mean(temp)), "min": float(np.min(temp)), "max": float(np.max(temp))},
         "safety_cap_events": {"vlimit": int(warn[0]), "force_cap": int(warn[1]), "dx_cap": int(warn[3])},
     }

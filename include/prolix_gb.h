@@ -101,13 +101,20 @@ typedef struct prx_system_desc {
   const double* exc_epsilon;
   const double* exc_chargeprod;
 
-  double cutoff;              /* A; 0 = no cutoff. LJ/Coulomb only, GB is never cut */
+  double cutoff;              /* A; 0 = no cutoff. LJ/Coulomb only unless gb_cutoff */
   double coulomb_constant;    /* 332.0637 (system.py:135) */
   double gb_coulomb_constant; /* constant of the GB prefactor */
   double solute_dielectric;   /* 1.0 */
   double solvent_dielectric;  /* 78.5 */
   double dielectric_offset;   /* 0.09 A */
   int32_t implicit_solvent;   /* 1: GB-OBC2 + ACE on */
+  int32_t gb_cutoff;          /* 1 (with cutoff > 0): the Born sums and the GB pair terms
+                                 also run over the pairs closer than the cutoff only --
+                                 the neighbour-list GB of generalized_born.py:266-415 as
+                                 the batched path calls it (batched_energy.py:999-1010),
+                                 with the list taken as every pair within the cutoff:
+                                 offset radii >= 1e-4, 1/B >= 1e-4, self term q^2/B.
+                                 0: GB over all pairs (physics/system.py:196-209). */
 } prx_system_desc;
 
 /* Langevin BAOAB parameters for one system (jax_md nvt_langevin as used at

@@ -34,6 +34,7 @@ def lib():
         L.oracle_cpu_create.restype = ctypes.c_void_p
         L.oracle_cpu_create.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         L.oracle_cpu_destroy.argtypes = [ctypes.c_void_p]
+        L.oracle_cpu_set_f32_cutoff.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.oracle_cpu_threads.restype = ctypes.c_int
         for name in ("oracle_cpu_energy_forces_f64", "oracle_cpu_energy_forces_f32"):
             getattr(L, name).argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
@@ -48,7 +49,9 @@ def lib():
 class CpuSystem:
     """One system on the CPU restatement (float64 or float32 arithmetic)."""
 
-    def __init__(self, sysd: dict, **opts):
+    def __init__(self, sysd: dict, f32_cutoff: bool = False, **opts):
+        """``f32_cutoff``: classify pairs against the cutoff with the float32
+        r^2 test of the HIP kernels (the arithmetic stays float64)."""
         from prolix_amd import _native as N  # ABI struct layout only (data marshalling)
 
         self._desc = N.Desc(sysd, **opts)
@@ -59,6 +62,7 @@ class CpuSystem:
         self._h = lib().oracle_cpu_create(ctypes.byref(self._desc.desc),
                                           None if coeff is None else coeff.ctypes.data_as(ctypes.c_void_p))
         self.n = self._desc.n_atoms
+        lib().oracle_cpu_set_f32_cutoff(self._h, 1 if f32_cutoff else 0)
 
     def energy_forces(self, pos, dtype=np.float64):
         x = np.ascontiguousarray(pos, dtype=dtype).reshape(-1, 3)

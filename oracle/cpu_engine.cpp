@@ -31,6 +31,11 @@ struct Sys {
   std::vector<double> bond_p, ub_p, angle_p, dih_p, imp_p, exc_s, exc_e, exc_q, coeff;
   std::vector<std::vector<std::pair<int, std::pair<double, double>>>> excl;  // per atom (j, (sv, se))
   int G = 0;
+  // 1: a pair is inside the cutoff iff the float32 test of the HIP kernels says
+  // so, fmaf(dx, dx, fmaf(dy, dy, dz * dz)) < (float)(cutoff^2) with float
+  // coordinate differences, so that pairs within float rounding of the cutoff
+  // fall on the same side as on the device; 0: the float64 test r^2 < cutoff^2
+  int f32_cut = 0;
 };
 
 template <class T>
@@ -112,16 +117,33 @@ void energy_forces(const Sys& S, const T* x, T* f, double* E) {
   std::vector<double> terms(PRX_N_ENERGY, 0.0);
   std::fill(f, f + 3 * n, (T)0);
   const T eps12 = (T)1e-12;
+  const T cut2 = d.cutoff > 0 ? (T)(d.cutoff * d.cutoff) : (T)0;
+  // neighbour-list GB (generalized_born.py:266-415): pairs closer than the cutoff
+  const bool nl = d.gb_cutoff && d.cutoff > 0;
+  const float cut2f = (float)(d.cutoff * d.cutoff);
+  auto inside = [&](int i, int j, T r2) {
+    if (!S.f32_cut) return r2 < cut2;
+    const float dx = (float)x[3 * j] - (float)x[3 * i], dy = (float)x[3 * j + 1] - (float)x[3 * i + 1],
+                dz = (float)x[3 * j + 2] - (float)x[3 * i + 2];
+    return std::fmaf(dx, dx, std::fmaf(dy, dy, dz * dz)) < cut2f;
+  };
+  auto gb_in = [&](int i, int j, T r2) { return !nl || inside(i, j, r2); };
+  auto rho_of = [&](int i) {
+    const double r = S.R[i] - d.dielectric_offset;
+    return (T)(nl ? std::max(std::max(S.R[i], 1e-6) - d.dielectric_offset, 1e-4) : r);
+  };
   // ---- Born sums
   std::vector<T> I(n, 0), B(n), dBdI(n), c(n);
 #pragma omp parallel for schedule(dynamic, 8)
   for (int i = 0; i < n; ++i) {
-    const T rho = (T)(S.R[i] - d.dielectric_offset);
+    const T rho = rho_of(i);
     T acc = 0;
     for (int j = 0; j < n; ++j) {
       if (j == i) continue;
       T dx[3] = {x[3 * i] - x[3 * j], x[3 * i + 1] - x[3 * j + 1], x[3 * i + 2] - x[3 * j + 2]};
-      const T r = std::sqrt(dot(dx, dx) + eps12);
+      const T r2p = dot(dx, dx);
+      if (!gb_in(i, j, r2p)) continue;
+      const T r = std::sqrt(r2p + eps12);
       acc += pair_integral(r, rho, (T)S.sr[j]);
     }
     I[i] = acc;
@@ -132,17 +154,25 @@ void energy_forces(const Sys& S, const T* x, T* f, double* E) {
   std::vector<double> dEdB(n, 0.0);
   double e_self = 0, e_ace = 0;
   for (int i = 0; i < n; ++i) {
-    const T rho = (T)(S.R[i] - d.dielectric_offset), Ri = (T)S.R[i];
+    const T rho = rho_of(i), Ri = (T)(nl ? std::max(S.R[i], 1e-4) : S.R[i]);
     const T psi = rho * I[i];
     const T t = psi - (T)0.8 * psi * psi + (T)4.85 * psi * psi * psi;
     const T th = std::tanh(t);
-    B[i] = (T)1 / ((T)1 / rho - th / Ri);
-    dBdI[i] = B[i] * B[i] * ((T)1 - th * th) * ((T)1 - (T)1.6 * psi + (T)14.55 * psi * psi) * rho / Ri;
-    const T e1 = std::exp(-eps12 / ((T)4 * B[i] * B[i]));
-    const T fii = std::sqrt(eps12 + B[i] * B[i] * e1);
+    T inv_b = (T)1 / rho - th / Ri;
+    const bool clamped = nl && inv_b < (T)1e-4;
+    if (clamped) inv_b = (T)1e-4;
+    B[i] = (T)1 / inv_b;
+    dBdI[i] = clamped ? (T)0 : B[i] * B[i] * ((T)1 - th * th) * ((T)1 - (T)1.6 * psi + (T)14.55 * psi * psi) * rho / Ri;
     const double qi = S.q[i];
-    e_self += pref * qi * qi / (double)fii;
-    dEdB[i] += -pref * qi * qi / (2.0 * (double)fii * fii * fii) * (double)(e1 * ((T)2 * B[i] + eps12 / ((T)2 * B[i])));
+    if (nl) {  // self term q^2 / B (generalized_born.py:412)
+      e_self += pref * qi * qi / (double)B[i];
+      dEdB[i] += -pref * qi * qi / ((double)B[i] * (double)B[i]);
+    } else {
+      const T e1 = std::exp(-eps12 / ((T)4 * B[i] * B[i]));
+      const T fii = std::sqrt(eps12 + B[i] * B[i] * e1);
+      e_self += pref * qi * qi / (double)fii;
+      dEdB[i] += -pref * qi * qi / (2.0 * (double)fii * fii * fii) * (double)(e1 * ((T)2 * B[i] + eps12 / ((T)2 * B[i])));
+    }
     const double rn = (S.R[i] - d.dielectric_offset) / 10.0 + 0.009;
     const double ace = 0.239006 * 28.3919551 * (rn + 0.14) * (rn + 0.14) * std::pow(rn / ((double)B[i] / 10.0), 6);
     if (d.implicit_solvent) {
@@ -154,7 +184,6 @@ void energy_forces(const Sys& S, const T* x, T* f, double* E) {
   std::vector<T> fth((size_t)nth * 3 * n, (T)0);
   std::vector<double> dth((size_t)nth * n, 0.0);
   std::vector<double> e_gb(nth, 0), e_cl(nth, 0), e_lj(nth, 0);
-  const T cut2 = d.cutoff > 0 ? (T)(d.cutoff * d.cutoff) : (T)0;
 #pragma omp parallel
   {
     const int tid = omp_get_thread_num();
@@ -168,7 +197,7 @@ void energy_forces(const Sys& S, const T* x, T* f, double* E) {
         T dx[3] = {x[3 * i] - x[3 * j], x[3 * i + 1] - x[3 * j + 1], x[3 * i + 2] - x[3 * j + 2]};
         const T r2 = dot(dx, dx);
         T fr = 0;  // -dE/dr / r
-        if (d.implicit_solvent) {
+        if (d.implicit_solvent && gb_in(i, j, r2)) {
           const T r2g = r2 + eps12;
           const T bb = B[i] * B[j];
           const T ex = std::exp(-r2g / ((T)4 * bb));
@@ -183,7 +212,7 @@ void energy_forces(const Sys& S, const T* x, T* f, double* E) {
           dt[j] += (double)(g * B[i]);
         }
         const T r = std::sqrt(r2);
-        const bool in = r > 0 && (cut2 == (T)0 || r2 < cut2);
+        const bool in = r > 0 && (cut2 == (T)0 || inside(i, j, r2));
         if (in) {
           const T scv = (T)sv[j], sce = (T)se[j];
           const T qq = (T)(S.q[i] * S.q[j]);
@@ -220,11 +249,13 @@ void energy_forces(const Sys& S, const T* x, T* f, double* E) {
       T* ft = &fth[(size_t)tid * 3 * n];
 #pragma omp for schedule(dynamic, 8)
       for (int i = 0; i < n; ++i) {
-        const T rhoi = (T)(S.R[i] - d.dielectric_offset), si = (T)S.sr[i];
+        const T rhoi = rho_of(i), si = (T)S.sr[i];
         for (int j = i + 1; j < n; ++j) {
           T dx[3] = {x[3 * i] - x[3 * j], x[3 * i + 1] - x[3 * j + 1], x[3 * i + 2] - x[3 * j + 2]};
-          const T r = std::sqrt(dot(dx, dx) + eps12);
-          const T rhoj = (T)(S.R[j] - d.dielectric_offset), sj = (T)S.sr[j];
+          const T r2p = dot(dx, dx);
+          if (!gb_in(i, j, r2p)) continue;
+          const T r = std::sqrt(r2p + eps12);
+          const T rhoj = rho_of(j), sj = (T)S.sr[j];
           const T g = (c[i] * pair_integral_dr(r, rhoi, sj) + c[j] * pair_integral_dr(r, rhoj, si)) / r;
           for (int k = 0; k < 3; ++k) { ft[3 * i + k] -= g * dx[k]; ft[3 * j + k] += g * dx[k]; }
         }
@@ -466,6 +497,8 @@ void* oracle_cpu_create(const prx_system_desc* d, const double* cmap_coeff) {
 }
 
 void oracle_cpu_destroy(void* h) { delete static_cast<Sys*>(h); }
+
+void oracle_cpu_set_f32_cutoff(void* h, int on) { static_cast<Sys*>(h)->f32_cut = on ? 1 : 0; }
 
 int oracle_cpu_threads(void) { return omp_get_max_threads(); }
 

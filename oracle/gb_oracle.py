@@ -24,6 +24,9 @@ under prolix's ``src/prolix/``):
 - ``exception_energy``         physics/bonded.py:188-228
 - ``cmap_coefficients`` / ``cmap_energy``   physics/cmap.py:62-389
 - ``energy_terms``             physics/system.py:229-248 (term assembly)
+- ``gb_cutoff`` options        physics/generalized_born.py:266-415 (neighbour-list
+                               Born radii and GB energy, the list taken as every
+                               pair closer than the cutoff)
 
 Forces are this module's own analytic derivatives of those energies. Tests pin
 them with central finite differences of the energies in float64, Newton's
@@ -102,36 +105,63 @@ def pair_integral_dr(r, rho_i, s_j):
     return np.where(r + s_j > rho_i, d1 + d2 + d3 + d4, 0.0)
 
 
-def born_sums(pos, radii, scaled_radii, offset=DIELECTRIC_OFFSET):
-    """I_i = sum_{j != i} I(r_ij; rho_i, sr_j) with r from safe_norm."""
+def _offset_radii(radii, offset, gb_cutoff):
+    radii = np.asarray(radii, np.float64)
+    if gb_cutoff:  # generalized_born.py:302, 311
+        return np.maximum(np.maximum(radii, 1e-6) - offset, 1e-4)
+    return radii - offset
+
+
+def _cut_mask(d, cutoff):
+    """Pairs inside the GB cutoff: plain distance r < cutoff (all pairs if cutoff <= 0)."""
+    if cutoff <= 0:
+        return np.ones(d.shape[:-1], bool)
+    return np.sum(d * d, axis=-1) < cutoff * cutoff
+
+
+def born_sums(pos, radii, scaled_radii, offset=DIELECTRIC_OFFSET, gb_cutoff=0.0):
+    """I_i = sum_{j != i} I(r_ij; rho_i, sr_j) with r from safe_norm; with
+    ``gb_cutoff`` > 0 only the pairs with r < gb_cutoff contribute."""
     pos = np.asarray(pos, np.float64)
     n = pos.shape[0]
-    rho = np.asarray(radii, np.float64) - offset
+    rho = _offset_radii(radii, offset, gb_cutoff > 0)
     sr = np.asarray(scaled_radii if scaled_radii is not None else radii, np.float64)
     out = np.zeros(n)
     for a, b in _row_blocks(n):
         d = pos[a:b, None, :] - pos[None, :, :]
         r = np.sqrt(np.sum(d * d, axis=-1) + SAFE_NORM_EPS)
-        I = pair_integral(r, rho[a:b, None], sr[None, :])
+        I = np.where(_cut_mask(d, gb_cutoff), pair_integral(r, rho[a:b, None], sr[None, :]), 0.0)
         idx = np.arange(a, b)
         I[idx - a, idx] = 0.0
         out[a:b] = I.sum(axis=1)
     return out
 
 
-def born_radii(pos, radii, scaled_radii=None, offset=DIELECTRIC_OFFSET, return_aux=False):
-    """OBC2 effective Born radii (generalized_born.py:32-94)."""
+def born_radii(pos, radii, scaled_radii=None, offset=DIELECTRIC_OFFSET, return_aux=False, gb_cutoff=0.0):
+    """OBC2 effective Born radii (generalized_born.py:32-94).
+
+    ``gb_cutoff`` > 0 gives the neighbour-list form (generalized_born.py:266-337)
+    with the list taken as every pair closer than gb_cutoff: radii clamped as
+    there, and 1/B floored at 1e-4.
+    """
     radii = np.asarray(radii, np.float64)
-    rho = radii - offset
-    I = born_sums(pos, radii, scaled_radii, offset)
+    nl = gb_cutoff > 0
+    rho = _offset_radii(radii, offset, nl)
+    Rd = np.maximum(np.maximum(radii, 1e-6), 1e-4) if nl else radii
+    I = born_sums(pos, radii, scaled_radii, offset, gb_cutoff)
     psi = rho * I
     t = ALPHA_OBC * psi - BETA_OBC * psi**2 + GAMMA_OBC * psi**3
     th = np.tanh(t)
-    inv_b = 1.0 / rho - th / radii
+    inv_b = 1.0 / rho - th / Rd
+    clamped = np.zeros_like(inv_b, bool)
+    if nl:
+        clamped = inv_b < 1e-4
+        inv_b = np.maximum(inv_b, 1e-4)
     B = 1.0 / inv_b
     if not return_aux:
         return B
-    dB_dI = B * B * (1.0 - th * th) * (ALPHA_OBC - 2 * BETA_OBC * psi + 3 * GAMMA_OBC * psi**2) * rho / radii
+    dB_dI = B * B * (1.0 - th * th) * (ALPHA_OBC - 2 * BETA_OBC * psi + 3 * GAMMA_OBC * psi**2) * rho / Rd
+    dB_dI = np.where(clamped, 0.0, dB_dI)
     return B, {"I": I, "psi": psi, "dB_dI": dB_dI}
 
 
@@ -143,8 +173,13 @@ def gb_prefactor(k_gb=COULOMB_CONSTANT, eps_in=1.0, eps_out=78.5):
     return -0.5 * k_gb * (1.0 / eps_in - 1.0 / eps_out)
 
 
-def gb_energy(pos, charges, B, k_gb=COULOMB_CONSTANT, eps_in=1.0, eps_out=78.5):
-    """E_GB over all ordered pairs including i == j (generalized_born.py:118-208)."""
+def gb_energy(pos, charges, B, k_gb=COULOMB_CONSTANT, eps_in=1.0, eps_out=78.5, gb_cutoff=0.0):
+    """E_GB over all ordered pairs including i == j (generalized_born.py:118-208).
+
+    With ``gb_cutoff`` > 0: the neighbour-list form (generalized_born.py:340-415),
+    sum over the ordered pairs i != j closer than gb_cutoff plus the self
+    term sum_i q_i^2 / B_i.
+    """
     pos = np.asarray(pos, np.float64)
     q = np.asarray(charges, np.float64)
     pref = gb_prefactor(k_gb, eps_in, eps_out)
@@ -154,7 +189,15 @@ def gb_energy(pos, charges, B, k_gb=COULOMB_CONSTANT, eps_in=1.0, eps_out=78.5):
         r2 = np.sum(d * d, axis=-1) + SAFE_NORM_EPS  # safe_norm squared
         bb = B[a:b, None] * B[None, :]
         f = np.sqrt(r2 + bb * np.exp(-r2 / (4.0 * bb)))
-        e += np.sum(q[a:b, None] * q[None, :] / f)
+        t = q[a:b, None] * q[None, :] / f
+        if gb_cutoff > 0:
+            m = _cut_mask(d, gb_cutoff)
+            idx = np.arange(a, b)
+            m[idx - a, idx] = False
+            t = np.where(m, t, 0.0)
+        e += np.sum(t)
+    if gb_cutoff > 0:
+        e += np.sum(q * q / B)
     return pref * e
 
 
@@ -426,8 +469,13 @@ def _get(sysd, *names):
 
 
 def energy_terms(sysd, pos, cutoff=0.0, k_c=COULOMB_CONSTANT, k_gb=COULOMB_CONSTANT, eps_in=1.0,
-                 eps_out=78.5, implicit_solvent=True, cmap_coeffs=None):
-    """Per-term energies of make_energy_fn(..., implicit_solvent=True) (system.py:229-248)."""
+                 eps_out=78.5, implicit_solvent=True, cmap_coeffs=None, gb_cutoff=False):
+    """Per-term energies of make_energy_fn(..., implicit_solvent=True) (system.py:229-248).
+
+    ``cutoff`` applies to LJ and Coulomb; with ``gb_cutoff`` (and cutoff > 0)
+    the Born sums and GB pair terms take the neighbour-list form over the
+    pairs closer than the cutoff (generalized_born.py:266-415, as called from
+    batched_energy.py:999-1010)."""
     pos = np.asarray(pos, np.float64)
     n = pos.shape[0]
     out = dict.fromkeys(TERM_NAMES, 0.0)
@@ -468,8 +516,9 @@ def energy_terms(sysd, pos, cutoff=0.0, k_c=COULOMB_CONSTANT, k_gb=COULOMB_CONST
     out["coulomb"] = coulomb_energy(pos, sysd["charges"], S_e, cutoff, k_c)
     if implicit_solvent:
         radii = np.asarray(sysd["radii"], np.float64)
-        B = born_radii(pos, radii, sysd.get("scaled_radii"))
-        out["gb"] = gb_energy(pos, sysd["charges"], B, k_gb, eps_in, eps_out)
+        gcut = float(cutoff) if gb_cutoff else 0.0
+        B = born_radii(pos, radii, sysd.get("scaled_radii"), gb_cutoff=gcut)
+        out["gb"] = gb_energy(pos, sysd["charges"], B, k_gb, eps_in, eps_out, gb_cutoff=gcut)
         out["ace"] = float(np.sum(ace_per_atom(radii, B)))
         out["_born_radii"] = B
     out["total"] = float(sum(v for k, v in out.items() if k in TERM_NAMES))
@@ -510,7 +559,7 @@ def _scatter_torsion(f, idx, dE_dphi, grad):
 
 
 def forces(sysd, pos, cutoff=0.0, k_c=COULOMB_CONSTANT, k_gb=COULOMB_CONSTANT, eps_in=1.0, eps_out=78.5,
-           implicit_solvent=True, cmap_coeffs=None, return_aux=False):
+           implicit_solvent=True, cmap_coeffs=None, return_aux=False, gb_cutoff=False):
     """F = -dE/dx of ``energy_terms`` (float64)."""
     pos = np.asarray(pos, np.float64)
     n = pos.shape[0]
@@ -612,14 +661,17 @@ def forces(sysd, pos, cutoff=0.0, k_c=COULOMB_CONSTANT, k_gb=COULOMB_CONSTANT, e
     if implicit_solvent:
         radii = np.asarray(sysd["radii"], np.float64)
         sr = np.asarray(sysd.get("scaled_radii") if sysd.get("scaled_radii") is not None else radii, np.float64)
-        rho = radii - DIELECTRIC_OFFSET
-        B, bx = born_radii(pos, radii, sr, return_aux=True)
+        gcut = float(cutoff) if gb_cutoff else 0.0
+        rho = _offset_radii(radii, DIELECTRIC_OFFSET, gcut > 0)
+        B, bx = born_radii(pos, radii, sr, return_aux=True, gb_cutoff=gcut)
         pref = gb_prefactor(k_gb, eps_in, eps_out)
         dEdB = np.zeros(n)
-        # self term, f_ii^2 = eps + B^2 exp(-eps / (4 B^2))
-        e1 = np.exp(-SAFE_NORM_EPS / (4.0 * B * B))
-        fii = np.sqrt(SAFE_NORM_EPS + B * B * e1)
-        dEdB += -pref * q * q / (2.0 * fii**3) * e1 * (2.0 * B + SAFE_NORM_EPS / (2.0 * B))
+        if gcut > 0:  # self term q^2 / B
+            dEdB += -pref * q * q / (B * B)
+        else:  # self term, f_ii^2 = eps + B^2 exp(-eps / (4 B^2))
+            e1 = np.exp(-SAFE_NORM_EPS / (4.0 * B * B))
+            fii = np.sqrt(SAFE_NORM_EPS + B * B * e1)
+            dEdB += -pref * q * q / (2.0 * fii**3) * e1 * (2.0 * B + SAFE_NORM_EPS / (2.0 * B))
         # ACE
         ace = ace_per_atom(radii, B)
         dEdB += -6.0 * ace / B
@@ -632,7 +684,7 @@ def forces(sysd, pos, cutoff=0.0, k_c=COULOMB_CONSTANT, k_gb=COULOMB_CONSTANT, e
             fgb = np.sqrt(r2 + bb * ex)
             qq = q[a:b, None] * q[None, :]
             idx = np.arange(a, b)
-            off = np.ones_like(r, bool)
+            off = _cut_mask(d, gcut)
             off[idx - a, idx] = False
             # direct part at fixed B: E = pref * qq / f over both orders -> 2 per unordered pair
             dEdr_over_r = np.where(off, 2.0 * pref * qq * (-(1.0 - ex / 4.0) / fgb**3), 0.0)
@@ -648,6 +700,7 @@ def forces(sysd, pos, cutoff=0.0, k_c=COULOMB_CONSTANT, k_gb=COULOMB_CONSTANT, e
             dIij = pair_integral_dr(r, rho[a:b, None], sr[None, :])
             dIji = pair_integral_dr(r, rho[None, :], sr[a:b, None])
             g = c[a:b, None] * dIij + c[None, :] * dIji
+            g = np.where(_cut_mask(d, gcut), g, 0.0)
             g[idx - a, idx] = 0.0
             f[a:b] -= np.sum((g / r)[:, :, None] * d, axis=1)
         aux = {"born_radii": B, "dE_dB": dEdB, "chain": c, **bx}

@@ -48,7 +48,7 @@ class SystemDesc(ctypes.Structure):
         ("cutoff", ctypes.c_double), ("coulomb_constant", ctypes.c_double),
         ("gb_coulomb_constant", ctypes.c_double), ("solute_dielectric", ctypes.c_double),
         ("solvent_dielectric", ctypes.c_double), ("dielectric_offset", ctypes.c_double),
-        ("implicit_solvent", ctypes.c_int32),
+        ("implicit_solvent", ctypes.c_int32), ("gb_cutoff", ctypes.c_int32),
     ]
 
 
@@ -143,7 +143,7 @@ class Desc:
     def __init__(self, sysd: dict, *, cutoff: float = 0.0, coulomb_constant: float = 332.0637,
                  gb_coulomb_constant: float = 332.0637, solute_dielectric: float = 1.0,
                  solvent_dielectric: float = 78.5, dielectric_offset: float = 0.09,
-                 implicit_solvent: bool = True):
+                 implicit_solvent: bool = True, gb_cutoff: bool = False):
         keep = {}
 
         def f64(name, tail=None):
@@ -250,6 +250,7 @@ class Desc:
         d.solvent_dielectric = float(solvent_dielectric)
         d.dielectric_offset = float(dielectric_offset)
         d.implicit_solvent = 1 if implicit_solvent else 0
+        d.gb_cutoff = 1 if gb_cutoff else 0
         self.desc = d
         self.keep = keep
         self.n_atoms = n

@@ -62,6 +62,7 @@ struct TileCtx {
   int g;       // this lane's atom (arena index)
   bool valid;  // lane < atoms in tile
   int slot, T, nT, base, lane;
+  bool skip_far;  // neighbour-list GB system: far items carry no partials
 };
 
 __device__ __forceinline__ int gather_wave() {
@@ -79,7 +80,12 @@ __device__ __forceinline__ TileCtx tile_ctx(const DeviceView& v, int t) {
   const SlotConst& S = v.slots[c.slot];
   c.nT = (S.n + kTile - 1) / kTile;
   c.base = S.item_base;
+  c.skip_far = S.gb_cut != 0;
   return c;
+}
+
+__device__ __forceinline__ bool item_off(const DeviceView& v, const TileCtx& c, int w) {
+  return c.skip_far && v.item_cls[w] == kClsFar;
 }
 
 __device__ __forceinline__ int row_start(const TileCtx& c, int I) { return c.base + I * c.nT - (I * (I - 1)) / 2; }
@@ -91,8 +97,14 @@ __device__ __forceinline__ double gather1(const DeviceView& v, const TileCtx& c)
   const float* pb = reinterpret_cast<const float*>(v.part);
   double acc = 0.0;
   const int r0 = row_start(c, c.T);
-  for (int J = c.T; J < c.nT; ++J) acc += (double)pb[pslot(r0 + (J - c.T), 0, c.lane)];
-  for (int I = 0; I <= c.T; ++I) acc += (double)pb[pslot(row_start(c, I) + (c.T - I), 1, c.lane)];
+  for (int J = c.T; J < c.nT; ++J) {
+    const int w = r0 + (J - c.T);
+    if (!item_off(v, c, w)) acc += (double)pb[pslot(w, 0, c.lane)];
+  }
+  for (int I = 0; I <= c.T; ++I) {
+    const int w = row_start(c, I) + (c.T - I);
+    if (!item_off(v, c, w)) acc += (double)pb[pslot(w, 1, c.lane)];
+  }
   return acc;
 }
 
@@ -100,12 +112,19 @@ __device__ __forceinline__ double4 gather4(const DeviceView& v, const TileCtx& c
   double4 acc = make_double4(0.0, 0.0, 0.0, 0.0);
   auto add = [&](float4 p) { acc.x += p.x; acc.y += p.y; acc.z += p.z; acc.w += p.w; };
   const int r0 = row_start(c, c.T);
-  for (int J = c.T; J < c.nT; ++J) add(v.part[pslot(r0 + (J - c.T), 0, c.lane)]);
-  for (int I = 0; I <= c.T; ++I) add(v.part[pslot(row_start(c, I) + (c.T - I), 1, c.lane)]);
+  for (int J = c.T; J < c.nT; ++J) {
+    const int w = r0 + (J - c.T);
+    if (!item_off(v, c, w)) add(v.part[pslot(w, 0, c.lane)]);
+  }
+  for (int I = 0; I <= c.T; ++I) {
+    const int w = row_start(c, I) + (c.T - I);
+    if (!item_off(v, c, w)) add(v.part[pslot(w, 1, c.lane)]);
+  }
   return acc;
 }
 
 // B_i from I_i in FP64 (generalized_born.py:81-93); dB/dI for the chain rule.
+// Neighbour-list GB systems floor 1/B at 1e-4 (generalized_born.py:333-337).
 __global__ __launch_bounds__(256) void k_born_gather(DeviceView v) {
   const int t = gather_wave();
   if (t >= v.n_tiles) return;
@@ -116,9 +135,11 @@ __global__ __launch_bounds__(256) void k_born_gather(DeviceView v) {
   const double psi = f.x * I;
   const double tt = psi * (1.0 + psi * (-0.8 + 4.85 * psi));
   const double th = tanh(tt);
-  const double inv_b = 1.0 / f.x - th / f.y;
+  double inv_b = 1.0 / f.x - th / f.y;
+  const bool floored = c.skip_far && inv_b < 1e-4;
+  if (floored) inv_b = 1e-4;
   const double B = 1.0 / inv_b;
-  const double dBdI = B * B * (1.0 - th * th) * (1.0 + psi * (-1.6 + 14.55 * psi)) * f.x / f.y;
+  const double dBdI = floored ? 0.0 : B * B * (1.0 - th * th) * (1.0 + psi * (-1.6 + 14.55 * psi)) * f.x / f.y;
   v.gb4[c.g] = make_float4((float)B, (float)(0.6005612043932249 / B), (float)dBdI, 0.0f);  // sqrt(log2(e)/4)/B
 }
 
@@ -402,6 +423,30 @@ __global__ void k_born_out(DeviceView v, float* out, const int32_t* map, int n) 
 static inline dim3 atom_grid(int n) { return dim3((n + 255) / 256); }
 
 static inline dim3 tile_grid(int n_tiles) { return dim3((n_tiles + 3) / 4); }
+
+// Bounding box of each tile's atoms at the current positions, one wave per tile.
+__global__ __launch_bounds__(256) void k_tile_boxes(DeviceView v) {
+  const int t = gather_wave();
+  if (t >= v.n_tiles) return;
+  const int4 tl = v.tiles[t];
+  const int lane = threadIdx.x & 63;
+  const float4 p = v.pos[tl.x + min(lane, tl.y - 1)];
+  float lx = p.x, ly = p.y, lz = p.z, hx = p.x, hy = p.y, hz = p.z;
+  for (int off = 32; off > 0; off >>= 1) {
+    lx = fminf(lx, __shfl_xor(lx, off));
+    ly = fminf(ly, __shfl_xor(ly, off));
+    lz = fminf(lz, __shfl_xor(lz, off));
+    hx = fmaxf(hx, __shfl_xor(hx, off));
+    hy = fmaxf(hy, __shfl_xor(hy, off));
+    hz = fmaxf(hz, __shfl_xor(hz, off));
+  }
+  if (lane == 0) v.tbox[2 * t] = make_float4(lx, ly, lz, 0.0f);
+  if (lane == 1) v.tbox[2 * t + 1] = make_float4(hx, hy, hz, 0.0f);
+}
+
+void launch_tile_boxes(const DeviceView& v, hipStream_t s) {
+  if (v.n_tiles) hipLaunchKernelGGL(k_tile_boxes, tile_grid(v.n_tiles), dim3(256), 0, s, v);
+}
 
 void launch_born_gather(const DeviceView& v, hipStream_t s) {
   if (v.n_tiles) hipLaunchKernelGGL(k_born_gather, tile_grid(v.n_tiles), dim3(256), 0, s, v);

@@ -32,6 +32,13 @@ struct PairItem {
 constexpr int kFlagDiag = 1;
 constexpr int kFlagExcl = 2;
 
+// Cutoff class of a work item, decided each evaluation from the bounding
+// boxes of its two tiles (a margin of 1e-5 in r^2 keeps the decision
+// conservative against float rounding of the per-pair test).
+constexpr int kClsNear = 0;      // every pair closer than the cutoff (or no cutoff)
+constexpr int kClsStraddle = 1;  // per-pair r^2 < cutoff^2 test needed
+constexpr int kClsFar = 2;       // every pair at or beyond the cutoff
+
 __host__ __device__ inline int32_t item_meta(int ni, int nj, int flags, int slot) {
   return ni | (nj << 7) | (flags << 14) | (slot << 16);
 }
@@ -48,7 +55,7 @@ struct SlotConst {
   float cutoff2;        // 0 = no cutoff
   float k_coulomb;      // Coulomb constant
   float gb2;            // 2 * GB prefactor, prefactor = -1/2 k_gb (1/eps_in - 1/eps_out)
-  float pad1;
+  int32_t gb_cut;       // 1: Born sums and GB pairs also restricted to r < cutoff
   double gb_pref;       // GB prefactor (FP64, per-atom self energy)
   float excl_v[4];      // class -> vdw scale (class 0 = 1)
   float excl_e[4];      // class -> elec scale
@@ -56,7 +63,7 @@ struct SlotConst {
   float dt, half_dt, c1, c2;   // c1 = exp(-gamma dt), c2 = sqrt(1 - c1^2)
   float sqrt_kT, kT;
   int32_t caps;         // 1: batched-step safety caps (velocity, displacement, force)
-  int32_t pad2;
+  int32_t tile0;        // global index of the system's first tile
   uint64_t seed;
 };
 
@@ -103,6 +110,9 @@ struct DeviceView {
   int32_t n_tiles;
   const unsigned long long* excl_masks;  // 128 per exclusion tile: lane * 2 + {bit0, bit1}
   double* e_items;     // 3 per item: GB pairs, Coulomb, LJ
+  float4* tbox;        // per tile t: bounding box lo (2t), hi (2t + 1) of its atoms
+  int32_t* item_cls;   // per item: cutoff class (systems with a cutoff only)
+  int32_t any_cut;     // 1 if some system has a cutoff (tile boxes are computed)
   // bonded
   BondEntry* bonds; int32_t bond_hw;
   AngleEntry* angles; int32_t angle_hw;
@@ -124,7 +134,8 @@ struct DeviceView {
 };
 
 // Launchers (defined in the .hip translation units).
-void launch_born_pairs(const DeviceView& v, hipStream_t s);
+void launch_tile_boxes(const DeviceView& v, hipStream_t s);                // -> tbox
+void launch_born_pairs(const DeviceView& v, hipStream_t s);                // also -> item_cls
 void launch_born_gather(const DeviceView& v, hipStream_t s);               // -> B, dB/dI
 void launch_force_pairs(const DeviceView& v, bool energy, hipStream_t s);
 void launch_force_gather(const DeviceView& v, bool energy, hipStream_t s);  // -> chain factors, pair force

@@ -220,6 +220,8 @@ struct prx_engine {
   float4* d_part = nullptr;   // 2 * 64 partials per item
   int4* d_tiles = nullptr;
   int64_t cap_tiles = 0;
+  float4* d_tbox = nullptr;
+  int32_t* d_item_cls = nullptr;
   double* d_e_items = nullptr;
   unsigned long long* d_excl = nullptr;
   float* d_coeff = nullptr;
@@ -307,7 +309,8 @@ struct prx_engine {
     void* ptrs[] = {v.pos, v.mom, v.force, v.born4, v.nb4, v.fin4, v.sqrt_m, v.slot_of, v.gb4, v.fpair, d_part, d_tiles,
                     v.e_atom, v.atom_step, v.bonds, v.angles, v.tors, v.cmaps, v.excs, v.e_bond, v.e_angle,
                     v.e_tors, v.e_cmap, v.e_exc, d_coeff, d_excl, v.slots, d_ranges, d_slot_list,
-                    d_energy_tmp, d_map, d_items, d_e_items, v.rec, v.atom_rec, d_rec_list, v.warn};
+                    d_energy_tmp, d_map, d_items, d_e_items, v.rec, v.atom_rec, d_rec_list, v.warn,
+                    d_tbox, d_item_cls};
     for (void* p : ptrs)
       if (p) hipFree(p);
     if (stream) hipStreamDestroy(stream);
@@ -372,10 +375,13 @@ struct prx_engine {
     std::vector<double4> fin4(n);
     std::vector<float> sqm(n);
     std::vector<int32_t> slot_of(n, sid);
+    // neighbour-list GB systems clamp the radii as generalized_born.py:302, 311, 331
+    const bool nl = d.gb_cutoff && d.cutoff > 0;
     for (int i = 0; i < n; ++i) {
       const double R = d.radii[i];
       const double sr = d.scaled_radii ? d.scaled_radii[i] : R;
-      const double rho = R - off;
+      const double rho = nl ? std::max(std::max(R, 1e-6) - off, 1e-4) : R - off;
+      const double R_tanh = nl ? std::max(R, 1e-4) : R;
       const double q = d.charges[i];
       const double sig = std::max(d.sigmas[i], 1e-6);
       const double m = d.masses ? d.masses[i] : 1.0;
@@ -386,7 +392,7 @@ struct prx_engine {
       const double aceK = d.implicit_solvent
                               ? 0.239006 * 28.3919551 * (rn + 0.14) * (rn + 0.14) * std::pow(10.0 * rn, 6)
                               : 0.0;
-      fin4[i] = make_double4(rho, R, q * q, aceK);
+      fin4[i] = make_double4(rho, R_tanh, q * q, aceK);
       sqm[i] = (float)std::sqrt(m);
       mom[i].w = (float)(1.0 / m);
     }
@@ -466,6 +472,7 @@ struct prx_engine {
     c.n = n;
     c.active = 1;
     c.cutoff2 = d.cutoff > 0 ? (float)(d.cutoff * d.cutoff) : 0.0f;
+    c.gb_cut = (d.gb_cutoff && d.cutoff > 0) ? 1 : 0;
     c.k_coulomb = (float)d.coulomb_constant;
     const double pref = d.implicit_solvent
                             ? -0.5 * d.gb_coulomb_constant * (1.0 / d.solute_dielectric - 1.0 / d.solvent_dielectric)
@@ -573,10 +580,13 @@ struct prx_engine {
     std::vector<int32_t> map;
     std::vector<int4> tiles;
     offsets.assign(1, 0);
+    int32_t any_cut = 0;
     for (int32_t sid : order) {
       Slot& s = slots[sid];
       const int32_t ib = (int32_t)items.size();
       s.c.item_base = ib;
+      s.c.tile0 = (int32_t)tiles.size();
+      if (s.c.cutoff2 > 0.0f) any_cut = 1;
       push_const(sid, s.c);
       const int nT = (s.n + prx::kTile - 1) / prx::kTile;
       for (int T = 0; T < nT; ++T)
@@ -595,14 +605,18 @@ struct prx_engine {
       if (d_e_items) HIPCHECK(hipFree(d_e_items));
       if (d_part) HIPCHECK(hipFree(d_part));
       cap_items = std::max<int64_t>((int64_t)items.size(), cap_items * 3 / 2);
+      if (d_item_cls) HIPCHECK(hipFree(d_item_cls));
       d_items = dalloc<prx::PairItem>(cap_items);
       d_e_items = dalloc<double>(cap_items * 3);
       d_part = dalloc<float4>(cap_items * 2 * prx::kTile);
+      d_item_cls = dalloc<int32_t>(cap_items);
     }
     if ((int64_t)tiles.size() > cap_tiles) {
       if (d_tiles) HIPCHECK(hipFree(d_tiles));
+      if (d_tbox) HIPCHECK(hipFree(d_tbox));
       cap_tiles = std::max<int64_t>((int64_t)tiles.size(), cap_tiles * 3 / 2);
       d_tiles = dalloc<int4>(cap_tiles);
+      d_tbox = dalloc<float4>(cap_tiles * 2);
     }
     if (!tiles.empty())
       HIPCHECK(hipMemcpy(d_tiles, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice));
@@ -617,6 +631,9 @@ struct prx_engine {
     v.part = d_part;
     v.tiles = d_tiles;
     v.n_tiles = (int32_t)tiles.size();
+    v.tbox = d_tbox;
+    v.item_cls = d_item_cls;
+    v.any_cut = any_cut;
     v.atom_hw = (int32_t)a_atoms.high_water();
     v.bond_hw = (int32_t)a_bond.high_water();
     v.angle_hw = (int32_t)a_ang.high_water();
@@ -632,6 +649,7 @@ struct prx_engine {
   // ---- evaluation -------------------------------------------------------
   // Everything of one force evaluation except the final chain gather.
   void force_passes(bool energy, hipStream_t s) {
+    if (v.any_cut) prx::launch_tile_boxes(v, s);
     prx::launch_born_pairs(v, s);
     prx::launch_born_gather(v, s);
     prx::launch_force_pairs(v, energy, s);
@@ -1061,6 +1079,7 @@ int prx_profile_passes(prx_engine* eng, int32_t n_reps, double* ms) {
         HIPCHECK(hipEventElapsedTime(&t, ev[k][0], ev[k][1]));
         ms[k] += t;
       };
+      if (eng->v.any_cut) prx::launch_tile_boxes(eng->v, s);
       timed(0, [&] { prx::launch_born_pairs(eng->v, s); });
       timed(1, [&] { prx::launch_born_gather(eng->v, s); });
       timed(2, [&] { prx::launch_force_pairs(eng->v, false, s); });

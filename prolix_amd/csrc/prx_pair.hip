@@ -19,6 +19,12 @@
 //
 // Diagonal tiles (I == J) visit each unordered pair once: offsets k = 1..31
 // on every lane and k = 32 on lanes 0..31 only.
+//
+// Systems with a cutoff: the Born pass classifies each item from its two
+// tiles' bounding boxes (near / straddle / far, stored for the later passes
+// and the gathers). Far items skip LJ and Coulomb, and with the neighbour-list
+// GB form (generalized_born.py:266-415, gb_cut) skip every pass; straddling
+// items test r^2 < cutoff^2 per pair.
 #include "prx_common.h"
 #include "prx_obc_farfield.h"
 
@@ -137,7 +143,8 @@ __device__ __forceinline__ float obc_far_dI(float s, float rinv) {
 // ---------------------------------------------------------------------------
 
 template <bool GENERAL, bool DIAG>
-__device__ __forceinline__ void born_tile(const DeviceView& v, const PairItem& it, int lane, int w) {
+__device__ __forceinline__ void born_tile(const DeviceView& v, const PairItem& it, int lane, int w,
+                                          float cut2) {
   const int ni = item_ni(it.meta), nj = item_nj(it.meta);
   const int gi = it.i0 + min(lane, ni - 1);
   const int gj = it.j0 + min(lane, nj - 1);
@@ -170,6 +177,7 @@ __device__ __forceinline__ void born_tile(const DeviceView& v, const PairItem& i
       const int orig = (lane + k) & 63;
       bool valid = ivalid && orig < nj;
       if (DIAG && k == 32) valid = valid && lane < 32;
+      if (cut2 > 0.0f) valid = valid && r2 < cut2;
       bp.I1 = valid ? bp.I1 : 0.0f;
       bp.I2 = valid ? bp.I2 : 0.0f;
     }
@@ -187,6 +195,25 @@ __device__ __forceinline__ void born_tile(const DeviceView& v, const PairItem& i
   pb[part_slot(w, 1, home)] = accJ;
 }
 
+// Cutoff class of an item from its tiles' bounding boxes (wave-uniform).
+__device__ __forceinline__ int classify(const DeviceView& v, const SlotConst& S, const PairItem& it) {
+  const int tI = S.tile0 + ((it.i0 - S.a0) >> 6);
+  const int tJ = S.tile0 + ((it.j0 - S.a0) >> 6);
+  const float4 li = v.tbox[2 * tI], hi = v.tbox[2 * tI + 1];
+  const float4 lj = v.tbox[2 * tJ], hj = v.tbox[2 * tJ + 1];
+  const float gx = fmaxf(0.0f, fmaxf(lj.x - hi.x, li.x - hj.x));
+  const float gy = fmaxf(0.0f, fmaxf(lj.y - hi.y, li.y - hj.y));
+  const float gz = fmaxf(0.0f, fmaxf(lj.z - hi.z, li.z - hj.z));
+  const float ex = fmaxf(hj.x - li.x, hi.x - lj.x);
+  const float ey = fmaxf(hj.y - li.y, hi.y - lj.y);
+  const float ez = fmaxf(hj.z - li.z, hi.z - lj.z);
+  const float dmin2 = gx * gx + gy * gy + gz * gz;
+  const float dmax2 = ex * ex + ey * ey + ez * ez;
+  if (dmin2 >= S.cutoff2 * 1.00001f) return kClsFar;
+  if (dmax2 < S.cutoff2 * 0.99999f) return kClsNear;
+  return kClsStraddle;
+}
+
 __global__ __launch_bounds__(256) void k_born_pairs(DeviceView v) {
   const int w = wave_id();
   if (w >= v.n_items) return;
@@ -194,12 +221,20 @@ __global__ __launch_bounds__(256) void k_born_pairs(DeviceView v) {
   const int lane = threadIdx.x & 63;
   const int fl = item_flags(it.meta);
   const bool full = item_ni(it.meta) == kTile && item_nj(it.meta) == kTile;
+  const SlotConst& S = v.slots[item_slot(it.meta)];
+  int cls = kClsNear;
+  if (S.cutoff2 > 0.0f) {
+    cls = classify(v, S, it);
+    if (lane == 0) v.item_cls[w] = cls;
+  }
+  if (S.gb_cut && cls == kClsFar) return;  // no partials: the gathers skip this item
+  const float cut2 = (S.gb_cut && cls == kClsStraddle) ? S.cutoff2 : 0.0f;
   if (fl & kFlagDiag) {
-    born_tile<true, true>(v, it, lane, w);
-  } else if (full) {
-    born_tile<false, false>(v, it, lane, w);
+    born_tile<true, true>(v, it, lane, w, cut2);
+  } else if (full && cut2 == 0.0f) {
+    born_tile<false, false>(v, it, lane, w, 0.0f);
   } else {
-    born_tile<true, false>(v, it, lane, w);
+    born_tile<true, false>(v, it, lane, w, cut2);
   }
 }
 
@@ -207,13 +242,18 @@ __global__ __launch_bounds__(256) void k_born_pairs(DeviceView v) {
 // Pass 2: GB + Coulomb + LJ
 // ---------------------------------------------------------------------------
 
-template <bool GENERAL, bool DIAG, bool ENERGY>
-__device__ __forceinline__ void force_tile(const DeviceView& v, const PairItem& it, int lane, int w) {
+// LJC = false: GB only (items entirely beyond the LJ/Coulomb cutoff).
+// GENERAL tiles test r^2 < cut2 per pair when cut2 > 0, on LJ/Coulomb and, if
+// gb_cut, on GB too.
+template <bool GENERAL, bool DIAG, bool ENERGY, bool LJC>
+__device__ __forceinline__ void force_tile(const DeviceView& v, const PairItem& it, int lane, int w, float cut2,
+                                           bool gb_cut) {
   const int ni = item_ni(it.meta), nj = item_nj(it.meta);
   const SlotConst& S = v.slots[item_slot(it.meta)];
-  const float kc = S.k_coulomb, gb2 = S.gb2, cut2 = S.cutoff2;
+  const float kc = S.k_coulomb, gb2 = S.gb2;
   const bool has_cut = GENERAL && cut2 > 0.0f;
-  const bool has_excl = GENERAL && it.excl >= 0;
+  const bool gb_has_cut = has_cut && gb_cut;
+  const bool has_excl = GENERAL && LJC && it.excl >= 0;
   const int gi = it.i0 + min(lane, ni - 1);
   const int gj = it.j0 + min(lane, nj - 1);
   const float4 pi = v.pos[gi], nbi = v.nb4[gi], gbi = v.gb4[gi];
@@ -241,13 +281,16 @@ __device__ __forceinline__ void force_tile(const DeviceView& v, const PairItem& 
     const float rinv2 = rinv * rinv;
     const float qq = nbi.x * qj;
     // Coulomb + LJ
-    float ec = kc * qq * rinv;
-    const float sg = nbi.y + hsj;
-    const float ep = nbi.z * sej;
-    const float s2 = sg * sg * rinv2;
-    const float s6 = s2 * s2 * s2;
-    float el = 4.0f * ep * (s6 * s6 - s6);
-    float fl = ep * s6 * fmaf(48.0f, s6, -24.0f);
+    float ec = 0.0f, el = 0.0f, fl = 0.0f;
+    if (LJC) {
+      ec = kc * qq * rinv;
+      const float sg = nbi.y + hsj;
+      const float ep = nbi.z * sej;
+      const float s2 = sg * sg * rinv2;
+      const float s6 = s2 * s2 * s2;
+      el = 4.0f * ep * (s6 * s6 - s6);
+      fl = ep * s6 * fmaf(48.0f, s6, -24.0f);
+    }
     // GB over every pair
     const float BB = gbi.x * Bj;
     const float x = r2 * (gbi.y * aj);  // r^2 log2(e) / (4 B_i B_j)
@@ -270,17 +313,18 @@ __device__ __forceinline__ void force_tile(const DeviceView& v, const PairItem& 
       }
       bool clv = valid && (sv != 0.0f || se != 0.0f);
       if (has_cut) clv = clv && r2 < cut2;
+      const bool gv = gb_has_cut ? (valid && r2 < cut2) : valid;
       ec = clv ? ec * se : 0.0f;
       el = clv ? el * sv : 0.0f;
       fl = clv ? fl * sv : 0.0f;
-      fgb = valid ? fgb : 0.0f;
-      g = valid ? g : 0.0f;
-      if (ENERGY) egb += valid ? (double)eg : 0.0;
+      fgb = gv ? fgb : 0.0f;
+      g = gv ? g : 0.0f;
+      if (ENERGY) egb += gv ? (double)eg : 0.0;
     } else {
       if (ENERGY) egb += (double)eg;
     }
-    if (ENERGY) { ecl += (double)ec; elj += (double)el; }
-    float fr = fmaf(ec + fl, rinv2, fgb);
+    if (ENERGY && LJC) { ecl += (double)ec; elj += (double)el; }
+    float fr = LJC ? fmaf(ec + fl, rinv2, fgb) : fgb;
     if (GENERAL) {
       // An idle lane of a ragged tile may pair an atom with itself (r = 0):
       // 0 * inf there is NaN, so the final scalar is selected, not just its parts.
@@ -326,13 +370,32 @@ __global__ __launch_bounds__(256) void k_force_pairs(DeviceView v) {
   const int lane = threadIdx.x & 63;
   const int fl = item_flags(it.meta);
   const bool full = item_ni(it.meta) == kTile && item_nj(it.meta) == kTile;
-  const bool plain = full && !(fl & kFlagExcl) && v.slots[item_slot(it.meta)].cutoff2 == 0.0f;
+  const SlotConst& S = v.slots[item_slot(it.meta)];
+  const int cls = S.cutoff2 > 0.0f ? v.item_cls[w] : kClsNear;
+  if (cls == kClsFar) {
+    if (S.gb_cut) {
+      if (ENERGY && lane == 0) {
+        v.e_items[(size_t)w * 3 + 0] = 0.0;
+        v.e_items[(size_t)w * 3 + 1] = 0.0;
+        v.e_items[(size_t)w * 3 + 2] = 0.0;
+      }
+      return;
+    }
+    // GB over every pair, nothing else (a far item is never diagonal)
+    if (full)
+      force_tile<false, false, ENERGY, false>(v, it, lane, w, 0.0f, false);
+    else
+      force_tile<true, false, ENERGY, false>(v, it, lane, w, 0.0f, false);
+    return;
+  }
+  const float cut2 = cls == kClsStraddle ? S.cutoff2 : 0.0f;
+  const bool gb_cut = S.gb_cut != 0;
   if (fl & kFlagDiag) {
-    force_tile<true, true, ENERGY>(v, it, lane, w);
-  } else if (plain) {
-    force_tile<false, false, ENERGY>(v, it, lane, w);
+    force_tile<true, true, ENERGY, true>(v, it, lane, w, cut2, gb_cut);
+  } else if (full && !(fl & kFlagExcl) && cut2 == 0.0f) {
+    force_tile<false, false, ENERGY, true>(v, it, lane, w, 0.0f, false);
   } else {
-    force_tile<true, false, ENERGY>(v, it, lane, w);
+    force_tile<true, false, ENERGY, true>(v, it, lane, w, cut2, gb_cut);
   }
 }
 
@@ -341,7 +404,8 @@ __global__ __launch_bounds__(256) void k_force_pairs(DeviceView v) {
 // ---------------------------------------------------------------------------
 
 template <bool GENERAL, bool DIAG>
-__device__ __forceinline__ void chain_tile(const DeviceView& v, const PairItem& it, int lane, int w) {
+__device__ __forceinline__ void chain_tile(const DeviceView& v, const PairItem& it, int lane, int w,
+                                           float cut2) {
   const int ni = item_ni(it.meta), nj = item_nj(it.meta);
   const int gi = it.i0 + min(lane, ni - 1);
   const int gj = it.j0 + min(lane, nj - 1);
@@ -376,6 +440,7 @@ __device__ __forceinline__ void chain_tile(const DeviceView& v, const PairItem& 
       const int orig = (lane + k) & 63;
       bool valid = ivalid && orig < nj;
       if (DIAG && k == 32) valid = valid && lane < 32;
+      if (cut2 > 0.0f) valid = valid && r2 < cut2;
       gsc = valid ? gsc : 0.0f;
     }
     fxi = fmaf(gsc, dx, fxi); fyi = fmaf(gsc, dy, fyi); fzi = fmaf(gsc, dz, fzi);
@@ -399,12 +464,19 @@ __global__ __launch_bounds__(256) void k_chain_pairs(DeviceView v) {
   const int lane = threadIdx.x & 63;
   const int fl = item_flags(it.meta);
   const bool full = item_ni(it.meta) == kTile && item_nj(it.meta) == kTile;
+  const SlotConst& S = v.slots[item_slot(it.meta)];
+  float cut2 = 0.0f;
+  if (S.gb_cut) {
+    const int cls = v.item_cls[w];
+    if (cls == kClsFar) return;
+    if (cls == kClsStraddle) cut2 = S.cutoff2;
+  }
   if (fl & kFlagDiag) {
-    chain_tile<true, true>(v, it, lane, w);
-  } else if (full) {
-    chain_tile<false, false>(v, it, lane, w);
+    chain_tile<true, true>(v, it, lane, w, cut2);
+  } else if (full && cut2 == 0.0f) {
+    chain_tile<false, false>(v, it, lane, w, 0.0f);
   } else {
-    chain_tile<true, false>(v, it, lane, w);
+    chain_tile<true, false>(v, it, lane, w, cut2);
   }
 }
 

@@ -53,14 +53,17 @@ class GBEngine:
     def __init__(self, systems, *, cutoff: float = 0.0, coulomb_constant: float = 332.0637,
                  gb_coulomb_constant: float = 332.0637, solute_dielectric: float = 1.0,
                  solvent_dielectric: float = 78.5, dielectric_offset: float = 0.09,
-                 implicit_solvent: bool = True):
+                 implicit_solvent: bool = True, gb_cutoff: bool = False):
+        """``cutoff`` (A, 0 = none) applies to LJ and Coulomb; with ``gb_cutoff``
+        the Born sums and GB pair terms also run over the pairs closer than the
+        cutoff only (the neighbour-list GB form)."""
         torch = _torch()
         self.torch = torch
         if isinstance(systems, dict):
             systems = [systems]
         opts = dict(cutoff=cutoff, coulomb_constant=coulomb_constant, gb_coulomb_constant=gb_coulomb_constant,
                     solute_dielectric=solute_dielectric, solvent_dielectric=solvent_dielectric,
-                    dielectric_offset=dielectric_offset, implicit_solvent=implicit_solvent)
+                    dielectric_offset=dielectric_offset, implicit_solvent=implicit_solvent, gb_cutoff=gb_cutoff)
         self._descs = [N.Desc(s, **opts) for s in systems]
         arr = (N.SystemDesc * len(self._descs))(*[d.desc for d in self._descs])
         h = ctypes.c_void_p()

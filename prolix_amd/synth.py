@@ -511,32 +511,43 @@ def _mix_member(args):
     return p
 
 
-def batch_mix(n_systems: int = 256, lo: int = 300, hi: int = 5000, seed: int = 2,
-              temperatures=(280.0, 350.0), workers: int | None = None):
-    """The uneven many-system mix: proteins plus per-system temperatures.
-
-    Systems are generated in a forked process pool (``workers`` processes, or
-    the CPUs this process may use, at most 16); call it before the process
-    touches the GPU. Each system depends only on its own seed, so the result is
-    the same for any worker count.
-    """
+def build_many(jobs, workers: int | None = None):
+    """Compact proteins for (n_atoms, seed) jobs, built in a forked process pool
+    (``workers`` processes, or the CPUs this process may use, at most 16); call
+    it before the process touches the GPU. Each system depends only on its own
+    seed, so the result is the same for any worker count."""
     import os
 
-    sizes = batch_mix_sizes(n_systems, lo, hi, seed)
-    rng = np.random.default_rng(seed + 7919)
-    temps = rng.uniform(temperatures[0], temperatures[1], size=n_systems)
-    jobs = [(int(n), seed * 100003 + k) for k, n in enumerate(sizes)]
+    jobs = [(int(n), int(sd)) for n, sd in jobs]
     if workers is None:
         workers = min(16, len(os.sched_getaffinity(0)))
-    if workers <= 1:
-        return [_mix_member(j) for j in jobs], temps
+    if workers <= 1 or len(jobs) <= 1:
+        return [_mix_member(j) for j in jobs]
     import multiprocessing as mp
 
     # largest first, so the pool's tail is short
     order = sorted(range(len(jobs)), key=lambda k: -jobs[k][0])
-    with mp.get_context("fork").Pool(workers) as pool:
+    with mp.get_context("fork").Pool(min(workers, len(jobs))) as pool:
         built = pool.map(_mix_member, [jobs[k] for k in order], chunksize=1)
     systems = [None] * len(jobs)
     for k, p in zip(order, built):
         systems[k] = p
-    return systems, temps
+    return systems
+
+
+def mix_jobs(n_systems: int = 256, lo: int = 300, hi: int = 5000, seed: int = 2):
+    """(n_atoms, seed) jobs of the uneven many-system mix."""
+    sizes = batch_mix_sizes(n_systems, lo, hi, seed)
+    return [(int(n), seed * 100003 + k) for k, n in enumerate(sizes)]
+
+
+def mix_temperatures(n_systems: int = 256, seed: int = 2, temperatures=(280.0, 350.0)) -> np.ndarray:
+    rng = np.random.default_rng(seed + 7919)
+    return rng.uniform(temperatures[0], temperatures[1], size=n_systems)
+
+
+def batch_mix(n_systems: int = 256, lo: int = 300, hi: int = 5000, seed: int = 2,
+              temperatures=(280.0, 350.0), workers: int | None = None):
+    """The uneven many-system mix: proteins plus per-system temperatures."""
+    systems = build_many(mix_jobs(n_systems, lo, hi, seed), workers)
+    return systems, mix_temperatures(n_systems, seed, temperatures)

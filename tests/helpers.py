@@ -41,3 +41,46 @@ def rel_rms(a, b) -> float:
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)
     return float(np.sqrt(np.mean((a - b) ** 2) / np.mean(b**2)))
+
+
+def _round_f32(q):
+    """Nearest float32 to the exact rational q (ties to even)."""
+    from fractions import Fraction
+
+    f = np.float32(float(q))
+    best = None
+    for c in (np.nextafter(f, np.float32(-np.inf)), f, np.nextafter(f, np.float32(np.inf))):
+        d = abs(Fraction(float(c)) - q)
+        key = (d, int(np.asarray(c).view(np.uint32)) & 1)
+        if best is None or key < best[0]:
+            best = (key, c)
+    return best[1]
+
+
+def kernel_r2_f32(d):
+    """r^2 of a float32 difference vector as the HIP pair kernels form it:
+    fmaf(dx, dx, fmaf(dy, dy, dz * dz)), each operation rounded once."""
+    from fractions import Fraction
+
+    dx, dy, dz = (np.float32(v) for v in d)
+    t = np.float32(dz * dz)
+    t = _round_f32(Fraction(float(dy)) ** 2 + Fraction(float(t)))
+    return _round_f32(Fraction(float(dx)) ** 2 + Fraction(float(t)))
+
+
+def boundary_pair(rc: float, seed: int = 0):
+    """A float32 difference vector whose exact length is >= rc while the
+    kernel's float32 r^2 is below (float)(rc^2): the pair is inside the cutoff
+    for the device and outside for an exact float64 test."""
+    from fractions import Fraction
+
+    rng = np.random.default_rng(seed)
+    c2 = np.float32(rc * rc)
+    for _ in range(100000):
+        u = rng.normal(size=3)
+        u /= np.linalg.norm(u)
+        d = (u * (rc + 5e-7)).astype(np.float32)
+        exact = sum(Fraction(float(v)) ** 2 for v in d)
+        if exact >= Fraction(rc) ** 2 and kernel_r2_f32(d) < c2:
+            return d
+    raise AssertionError("no boundary pair found")

@@ -166,7 +166,7 @@ def test_harmonic_impropers_exceptions_ub():
 
 
 def test_deterministic_and_batch_equals_single():
-    """Fixed-point accumulation: repeated calls and batched vs alone are bitwise equal."""
+    """Fixed-order accumulation: repeated calls and batched vs alone are bitwise equal."""
     from prolix_amd.engine import GBEngine
 
     systems = [protein(n, 20 + k, radius=synth.compact_radius(n)) for k, n in enumerate((97, 300, 513))]

@@ -172,3 +172,26 @@ def test_cpu_restatement_zero_friction_is_velocity_verlet():
     for _ in range(5):
         x0, p0, f0 = O.baoab_step(x0, p0, f0, m, dt, 0.0, 0.0, np.zeros_like(x0), ff)
     assert np.max(np.abs(x - x0)) < 1e-10
+
+
+def test_cpu_restatement_float32_cutoff_classification():
+    """With f32_cutoff the restatement puts a pair on the cutoff's side that the
+    kernels' float32 r^2 test picks (emulated here with exact rationals); with
+    it off, on the float64 side. Away from the cutoff both modes agree."""
+    from oracle import cpu as C
+    from tests.golden import make_golden as G
+    from tests.helpers import boundary_pair
+
+    rc = 12.0
+    d = boundary_pair(rc)
+    s = {"positions": np.array([[0.0, 0.0, 0.0], d.astype(np.float64)]), "charges": np.array([1.0, -1.0]),
+         "sigmas": np.array([2.0, 2.0]), "epsilons": np.array([0.1, 0.1]), "radii": np.array([1.5, 1.5])}
+    e32, _ = C.CpuSystem(s, cutoff=rc, f32_cutoff=True).energy_forces(s["positions"])
+    e64, _ = C.CpuSystem(s, cutoff=rc).energy_forces(s["positions"])
+    assert e64[N.TERMS.index("coulomb")] == 0.0
+    assert e32[N.TERMS.index("coulomb")] == pytest.approx(-332.0637 / rc, rel=1e-6)
+    sysd, cutoff, gbc, *_ = G.load("peptide138_gbcut12")
+    a, fa = C.CpuSystem(sysd, cutoff=cutoff, gb_cutoff=gbc, f32_cutoff=True).energy_forces(sysd["positions"])
+    b, fb = C.CpuSystem(sysd, cutoff=cutoff, gb_cutoff=gbc).energy_forces(sysd["positions"])
+    # (the OpenMP restatement's sums are not in a fixed order: equal to rounding)
+    assert np.allclose(a, b, rtol=1e-13, atol=1e-10) and np.allclose(fa, fb, rtol=1e-12, atol=1e-11)

@@ -228,3 +228,37 @@ def test_box_muller_stream_statistics():
     xs = np.array([philox.normal3(12345, s, i) for s in range(40) for i in range(100)]).ravel()
     assert abs(xs.mean()) < 0.05
     assert abs(xs.std() - 1.0) < 0.05
+
+
+# ------------------------------------------------------- neighbour-list GB form
+
+def test_gb_cutoff_with_complete_list_equals_dense():
+    """tests/test_neighbor_list_energy.py:73-100: a list holding every pair gives
+    the dense energy. Cutoff above the largest distance -> only the self-term
+    form (q^2/B vs q^2/f_ii with the safe_norm eps) differs, by ~1e-13."""
+    s = protein(60, 9)
+    x = s["positions"]
+    rmax = np.max(np.linalg.norm(x[:, None] - x[None], axis=-1))
+    dense = O.energy_terms(s, x, cutoff=rmax + 1.0)
+    nl = O.energy_terms(s, x, cutoff=rmax + 1.0, gb_cutoff=True)
+    assert nl["gb"] == pytest.approx(dense["gb"], rel=1e-11)
+    assert np.allclose(nl["_born_radii"], dense["_born_radii"], rtol=1e-13)
+    fd = O.forces(s, x, cutoff=rmax + 1.0)
+    fn = O.forces(s, x, cutoff=rmax + 1.0, gb_cutoff=True)
+    assert np.max(np.abs(fd - fn)) < 1e-9 * np.max(np.abs(fd))
+
+
+def test_gb_cutoff_separated_ions_are_isolated():
+    """Ions farther apart than the cutoff: each has B = R - 0.09 and
+    E_GB = pref * sum q^2 / B (generalized_born.py:333-337, 410-414)."""
+    s = {"charges": np.array([1.0, -0.5]), "sigmas": np.ones(2), "epsilons": np.zeros(2),
+         "radii": np.array([1.7, 2.1])}
+    x = np.array([[0.0, 0, 0], [6.0, 0, 0]])
+    t = O.energy_terms(s, x, cutoff=5.0, gb_cutoff=True)
+    B = np.array([1.61, 2.01])
+    assert np.allclose(t["_born_radii"], B, rtol=1e-14)
+    assert t["gb"] == pytest.approx(O.gb_prefactor() * np.sum(s["charges"] ** 2 / B), rel=1e-13)
+    # the dense form still sees the partner
+    assert O.energy_terms(s, x, cutoff=5.0)["gb"] != pytest.approx(t["gb"], rel=1e-6)
+    f = O.forces(s, x, cutoff=5.0, gb_cutoff=True)
+    assert np.max(np.abs(f)) == 0.0
