@@ -51,28 +51,29 @@ __device__ __forceinline__ float3 normal3(uint64_t seed, uint32_t step, int loca
 }
 
 // ---------------------------------------------------------------------------
-// Tile gathers: one wave per 64-atom tile T of a system with nT tiles. The
-// tile's atoms received i-side partials from the row items (T, J), J >= T,
-// and j-side partials from the column items (I, T), I <= T; items are stored
-// row-major per system, so item (I, J) = base + I nT - I (I - 1) / 2 + (J - I).
-// Sums run in that fixed order, in FP64.
+// Tile gathers: one 256-thread block (four waves) per 64-atom tile T of a
+// system with nT tiles. The tile's atoms received i-side partials from the
+// row items (T, J), J >= T, and j-side partials from the column items
+// (I, T), I <= T; items are stored row-major per system, so item (I, J) =
+// base + I nT - I (I - 1) / 2 + (J - I). Wave q sums the row items
+// J = T + q + 4n and the column items I = q + 4n in that order, in FP64; the
+// four wave sums are then added in the order q = 0, 1, 2, 3 through LDS, so
+// every result is a fixed function of the item partials.
 // ---------------------------------------------------------------------------
+constexpr int kGW = 4;  // waves per tile gather
 
 struct TileCtx {
   int g;       // this lane's atom (arena index)
   bool valid;  // lane < atoms in tile
-  int slot, T, nT, base, lane;
+  int slot, T, nT, base, lane, q;
   bool skip_far;  // neighbour-list GB system: far items carry no partials
 };
-
-__device__ __forceinline__ int gather_wave() {
-  return __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)));
-}
 
 __device__ __forceinline__ TileCtx tile_ctx(const DeviceView& v, int t) {
   const int4 tl = v.tiles[t];
   TileCtx c;
   c.lane = threadIdx.x & 63;
+  c.q = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   c.g = tl.x + c.lane;
   c.valid = c.lane < tl.y;
   c.slot = tl.z;
@@ -84,53 +85,85 @@ __device__ __forceinline__ TileCtx tile_ctx(const DeviceView& v, int t) {
   return c;
 }
 
-__device__ __forceinline__ bool item_off(const DeviceView& v, const TileCtx& c, int w) {
-  return c.skip_far && v.item_cls[w] == kClsFar;
-}
-
 __device__ __forceinline__ int row_start(const TileCtx& c, int I) { return c.base + I * c.nT - (I * (I - 1)) / 2; }
 
 __device__ __forceinline__ size_t pslot(int w, int side, int lane) { return ((size_t)w * 2 + side) * kTile + lane; }
 
-// Scalar partials (Born sums) of this lane's atom.
+// Calls add(item, side) for this wave's items in their fixed order. Systems
+// whose far items carry no partials test 64 candidates per vector load of
+// the item classes and visit only the active ones.
+template <class F>
+__device__ __forceinline__ void for_tile_items(const DeviceView& v, const TileCtx& c, F&& add) {
+  const int r0 = row_start(c, c.T);
+  const int n_row = max(0, (c.nT - c.T - c.q + kGW - 1) / kGW);
+  const int n_col = max(0, (c.T + 1 - c.q + kGW - 1) / kGW);
+  auto col_item = [&](int n) {
+    const int I = c.q + kGW * n;
+    return row_start(c, I) + (c.T - I);
+  };
+  if (!c.skip_far) {
+    for (int n = 0; n < n_row; ++n) add(r0 + c.q + kGW * n, 0);
+    for (int n = 0; n < n_col; ++n) add(col_item(n), 1);
+    return;
+  }
+  for (int g0 = 0; g0 < n_row; g0 += 64) {
+    const int n = g0 + c.lane;
+    const bool act = n < n_row && v.item_cls[r0 + c.q + kGW * n] != kClsFar;
+    for (unsigned long long m = __ballot(act); m; m &= m - 1) add(r0 + c.q + kGW * (g0 + __builtin_ctzll(m)), 0);
+  }
+  for (int g0 = 0; g0 < n_col; g0 += 64) {
+    const int n = g0 + c.lane;
+    const bool act = n < n_col && v.item_cls[col_item(min(n, n_col - 1))] != kClsFar;
+    for (unsigned long long m = __ballot(act); m; m &= m - 1) add(col_item(g0 + __builtin_ctzll(m)), 1);
+  }
+}
+
+// Scalar partials (Born sums) of this wave's items for the lane's atom.
 __device__ __forceinline__ double gather1(const DeviceView& v, const TileCtx& c) {
   const float* pb = reinterpret_cast<const float*>(v.part);
   double acc = 0.0;
-  const int r0 = row_start(c, c.T);
-  for (int J = c.T; J < c.nT; ++J) {
-    const int w = r0 + (J - c.T);
-    if (!item_off(v, c, w)) acc += (double)pb[pslot(w, 0, c.lane)];
-  }
-  for (int I = 0; I <= c.T; ++I) {
-    const int w = row_start(c, I) + (c.T - I);
-    if (!item_off(v, c, w)) acc += (double)pb[pslot(w, 1, c.lane)];
-  }
+  for_tile_items(v, c, [&](int w, int side) { acc += (double)pb[pslot(w, side, c.lane)]; });
   return acc;
 }
 
 __device__ __forceinline__ double4 gather4(const DeviceView& v, const TileCtx& c) {
   double4 acc = make_double4(0.0, 0.0, 0.0, 0.0);
-  auto add = [&](float4 p) { acc.x += p.x; acc.y += p.y; acc.z += p.z; acc.w += p.w; };
-  const int r0 = row_start(c, c.T);
-  for (int J = c.T; J < c.nT; ++J) {
-    const int w = r0 + (J - c.T);
-    if (!item_off(v, c, w)) add(v.part[pslot(w, 0, c.lane)]);
-  }
-  for (int I = 0; I <= c.T; ++I) {
-    const int w = row_start(c, I) + (c.T - I);
-    if (!item_off(v, c, w)) add(v.part[pslot(w, 1, c.lane)]);
-  }
+  for_tile_items(v, c, [&](int w, int side) {
+    const float4 p = v.part[pslot(w, side, c.lane)];
+    acc.x += p.x; acc.y += p.y; acc.z += p.z; acc.w += p.w;
+  });
   return acc;
+}
+
+// The four waves' sums, added in wave order; valid in wave 0 after the call.
+// Every thread of the block must call it (it holds a barrier).
+__device__ __forceinline__ double combine1(double x, const TileCtx& c) {
+  __shared__ double sh[kGW][kTile];
+  sh[c.q][c.lane] = x;
+  __syncthreads();
+  return ((sh[0][c.lane] + sh[1][c.lane]) + sh[2][c.lane]) + sh[3][c.lane];
+}
+
+__device__ __forceinline__ double4 combine4(double4 x, const TileCtx& c) {
+  __shared__ double4 sh4[kGW][kTile];
+  sh4[c.q][c.lane] = x;
+  __syncthreads();
+  double4 r = sh4[0][c.lane];
+#pragma unroll
+  for (int k = 1; k < kGW; ++k) {
+    const double4 y = sh4[k][c.lane];
+    r.x += y.x; r.y += y.y; r.z += y.z; r.w += y.w;
+  }
+  return r;
 }
 
 // B_i from I_i in FP64 (generalized_born.py:81-93); dB/dI for the chain rule.
 // Neighbour-list GB systems floor 1/B at 1e-4 (generalized_born.py:333-337).
 __global__ __launch_bounds__(256) void k_born_gather(DeviceView v) {
-  const int t = gather_wave();
-  if (t >= v.n_tiles) return;
-  const TileCtx c = tile_ctx(v, t);
-  const double I = gather1(v, c);
-  if (!c.valid) return;
+  if ((int)blockIdx.x >= v.n_tiles) return;
+  const TileCtx c = tile_ctx(v, blockIdx.x);
+  const double I = combine1(gather1(v, c), c);
+  if (c.q != 0 || !c.valid) return;
   const double4 f = v.fin4[c.g];  // rho, R, q^2, ace K
   const double psi = f.x * I;
   const double tt = psi * (1.0 + psi * (-0.8 + 4.85 * psi));
@@ -146,11 +179,10 @@ __global__ __launch_bounds__(256) void k_born_gather(DeviceView v) {
 // Pair force and dE/dB; c_i = dE/dB_i * dB_i/dI_i with the GB self term and ACE (FP64).
 template <bool ENERGY>
 __global__ __launch_bounds__(256) void k_force_gather(DeviceView v) {
-  const int t = gather_wave();
-  if (t >= v.n_tiles) return;
-  const TileCtx c = tile_ctx(v, t);
-  const double4 a = gather4(v, c);
-  if (!c.valid) return;
+  if ((int)blockIdx.x >= v.n_tiles) return;
+  const TileCtx c = tile_ctx(v, blockIdx.x);
+  const double4 a = combine4(gather4(v, c), c);
+  if (c.q != 0 || !c.valid) return;
   const double4 f = v.fin4[c.g];
   const float4 gb = v.gb4[c.g];
   const double B = (double)gb.x;
@@ -166,11 +198,11 @@ __global__ __launch_bounds__(256) void k_force_gather(DeviceView v) {
   }
 }
 
-// Bonded records of atom g, summed in the fixed order of its record list.
-__device__ __forceinline__ float3 bonded_force(const DeviceView& v, int g) {
+// Bonded records q, q + 4, ... of atom g (wave q's share), in list order.
+__device__ __forceinline__ float3 bonded_force(const DeviceView& v, int g, int q) {
   const int2 lr = v.atom_rec[g];
   float bx = 0.0f, by = 0.0f, bz = 0.0f;
-  for (int k = 0; k < lr.y; ++k) {
+  for (int k = q; k < lr.y; k += kGW) {
     const float4 r = v.rec[v.rec_list[lr.x + k]];
     bx += r.x;
     by += r.y;
@@ -224,13 +256,22 @@ __device__ __forceinline__ void begin_step(const DeviceView& v, const SlotConst&
 // Total force = chain partials + pair force + bonded records; then
 //   mode -1: store it; 2: B(F) and count the step; 1: B(F), then B A O A.
 __global__ __launch_bounds__(256) void k_chain_gather(DeviceView v, int mode) {
-  const int t = gather_wave();
-  if (t >= v.n_tiles) return;
-  const TileCtx c = tile_ctx(v, t);
-  const double4 a = gather4(v, c);
-  if (!c.valid) return;
+  if ((int)blockIdx.x >= v.n_tiles) return;
+  const TileCtx c = tile_ctx(v, blockIdx.x);
+  double4 part = gather4(v, c);
+  float3 fb = c.valid ? bonded_force(v, c.g, c.q) : make_float3(0.f, 0.f, 0.f);
+  // the bonded share travels in the same combine (its float sum is exact in FP64)
+  __shared__ float4 shb[kGW][kTile];
+  shb[c.q][c.lane] = make_float4(fb.x, fb.y, fb.z, 0.0f);
+  const double4 a = combine4(part, c);
+  if (c.q != 0 || !c.valid) return;
+  fb = make_float3(0.f, 0.f, 0.f);
+#pragma unroll
+  for (int k = 0; k < kGW; ++k) {
+    const float4 y = shb[k][c.lane];
+    fb.x += y.x; fb.y += y.y; fb.z += y.z;
+  }
   const float4 fp = v.fpair[c.g];
-  const float3 fb = bonded_force(v, c.g);
   float3 f = make_float3((float)(a.x + (double)fp.x) + fb.x, (float)(a.y + (double)fp.y) + fb.y,
                          (float)(a.z + (double)fp.z) + fb.z);
   const SlotConst& S = v.slots[c.slot];
@@ -422,7 +463,12 @@ __global__ void k_born_out(DeviceView v, float* out, const int32_t* map, int n) 
 
 static inline dim3 atom_grid(int n) { return dim3((n + 255) / 256); }
 
-static inline dim3 tile_grid(int n_tiles) { return dim3((n_tiles + 3) / 4); }
+static inline dim3 tile_grid(int n_tiles) { return dim3((n_tiles + 3) / 4); }  // one wave per tile
+static inline dim3 gather_grid(int n_tiles) { return dim3(n_tiles); }          // one block per tile
+
+__device__ __forceinline__ int gather_wave() {
+  return __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)));
+}
 
 // Bounding box of each tile's atoms at the current positions, one wave per tile.
 __global__ __launch_bounds__(256) void k_tile_boxes(DeviceView v) {
@@ -449,17 +495,17 @@ void launch_tile_boxes(const DeviceView& v, hipStream_t s) {
 }
 
 void launch_born_gather(const DeviceView& v, hipStream_t s) {
-  if (v.n_tiles) hipLaunchKernelGGL(k_born_gather, tile_grid(v.n_tiles), dim3(256), 0, s, v);
+  if (v.n_tiles) hipLaunchKernelGGL(k_born_gather, gather_grid(v.n_tiles), dim3(256), 0, s, v);
 }
 void launch_force_gather(const DeviceView& v, bool energy, hipStream_t s) {
   if (!v.n_tiles) return;
   if (energy)
-    hipLaunchKernelGGL(k_force_gather<true>, tile_grid(v.n_tiles), dim3(256), 0, s, v);
+    hipLaunchKernelGGL(k_force_gather<true>, gather_grid(v.n_tiles), dim3(256), 0, s, v);
   else
-    hipLaunchKernelGGL(k_force_gather<false>, tile_grid(v.n_tiles), dim3(256), 0, s, v);
+    hipLaunchKernelGGL(k_force_gather<false>, gather_grid(v.n_tiles), dim3(256), 0, s, v);
 }
 void launch_chain_gather(const DeviceView& v, int mode, hipStream_t s) {
-  if (v.n_tiles) hipLaunchKernelGGL(k_chain_gather, tile_grid(v.n_tiles), dim3(256), 0, s, v, mode);
+  if (v.n_tiles) hipLaunchKernelGGL(k_chain_gather, gather_grid(v.n_tiles), dim3(256), 0, s, v, mode);
 }
 void launch_step_begin(const DeviceView& v, hipStream_t s) {
   if (v.atom_hw) hipLaunchKernelGGL(k_step_begin, atom_grid(v.atom_hw), dim3(256), 0, s, v);

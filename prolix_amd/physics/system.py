@@ -26,9 +26,17 @@ Parameter resolution (reference line in parentheses):
 - explicit 1-4 exceptions come from ``exclusion_spec.exception_*``;
 - GB uses 78.5 / 1.0 dielectrics and the Coulomb constant 332.0637.
 
-Not supported (raise): periodic boxes / PME (``use_pbc``), neighbour lists
-(``neighbor`` argument) -- explicit solvent and the neighbour-list path are
-outside this engine.
+- ``neighbor`` (a jax_md neighbour list or its ``idx``) is accepted and its
+  contents are not read: with the list, the reference restricts LJ and
+  Coulomb to listed pairs closer than the cutoff (optimization.py:128-194,
+  299-357) and keeps GB dense (system.py:196-216), which is the value
+  computed here for every list that holds all pairs within the cutoff;
+- ``gb_cutoff=True`` (keyword) selects the neighbour-list GB form of the
+  batched path (generalized_born.py:266-415, batched_energy.py:999-1010):
+  Born sums and GB pair terms over the pairs closer than the cutoff.
+
+Not supported (raise): periodic boxes / PME (``use_pbc``, ``box``), the
+explicit-solvent path.
 """
 
 from __future__ import annotations
@@ -126,8 +134,6 @@ class _EnergyFn:
         self._last = None
 
     def _eval(self, r, neighbor=None):
-        if neighbor is not None:
-            raise NotImplementedError("neighbour-list evaluation is not part of this engine; pass neighbor=None")
         key = None
         if not hasattr(r, "data_ptr"):
             a = np.asarray(r, np.float32)
@@ -177,7 +183,7 @@ def make_energy_fn(displacement_fn, system, cutoff_distance: float = 9.0, return
         sysd["exception_chargeprods"] = _np(exclusion_spec.exception_chargeprods)
     cutoff = float(kwargs.get("cutoff", cutoff_distance) or 0.0)
     engine = GBEngine(sysd, cutoff=cutoff, coulomb_constant=COULOMB_CONSTANT, gb_coulomb_constant=COULOMB_CONSTANT,
-                      implicit_solvent=bool(implicit_solvent))
+                      implicit_solvent=bool(implicit_solvent), gb_cutoff=bool(kwargs.get("gb_cutoff", False)))
     fn = _EnergyFn(engine, n)
     fn.system = sysd
     if not return_decomposed:

@@ -36,7 +36,14 @@ def _relaxed(sysd, steps=100):
 
 
 def test_zero_friction_matches_float64_verlet():
-    """gamma = 0 reduces BAOAB to velocity Verlet (settle.py:2064-2087)."""
+    """gamma = 0 reduces BAOAB to velocity Verlet (settle.py:2064-2087).
+
+    Each device step is compared with one float64 Verlet step taken from the
+    device's own previous state (float64 oracle forces): positions agree to a
+    few float32 ulps and momenta to float32 resolution. Over many steps two
+    float32 runs of a chaotic system separate at a rate set by their rounding,
+    so the 50-step trajectory is only bounded loosely against float64 Verlet.
+    """
     from prolix_amd.engine import Langevin
 
     print("[gpu] zero-friction trajectory, 138 atoms", flush=True)
@@ -47,37 +54,41 @@ def test_zero_friction_matches_float64_verlet():
     p0 = (rng.normal(size=(n, 3)) * np.sqrt(m[:, None] * KB * 300.0)).astype(np.float32).astype(np.float64)
     x0 = sysd["positions"]
     steps, dt_fs = 50, 1.0
-    eng = _engine(sysd)
-    eng.init(x0, Langevin(dt_fs=dt_fs, temperature=300.0, gamma_ps=0.0, seed=5), momenta=p0)
-    eng.steps(steps)
-    xg, pg, fg = (a.cpu().numpy().astype(np.float64) for a in eng.state())
-    eng.close()
     dt = dt_fs / AKMA_FS
     ff = lambda x: O.forces(sysd, x)
 
-    def verlet(round32):
-        r32 = (lambda a: a.astype(np.float32).astype(np.float64)) if round32 else (lambda a: a)
-        x, p, f = x0.copy(), p0.copy(), ff(x0)
-        for _ in range(steps):
-            p = r32(p + 0.5 * dt * f)
-            x = r32(x + 0.5 * dt * p / m[:, None])
-            x = r32(x + 0.5 * dt * p / m[:, None])
-            f = r32(ff(x))
-            p = r32(p + 0.5 * dt * f)
-        return x, p
+    def verlet_step(x, p, f):
+        p = p + 0.5 * dt * f
+        x = x + dt * p / m[:, None]
+        f = ff(x)
+        return x, p + 0.5 * dt * f, f
 
-    x, p = verlet(False)
-    x32, p32 = verlet(True)  # the same integration with float32 state
-    dx = np.sqrt(np.mean((xg - x) ** 2))
-    dp = np.sqrt(np.mean((pg - p) ** 2) / np.mean(p**2))
-    dx32 = np.sqrt(np.mean((x32 - x) ** 2))
-    dp32 = np.sqrt(np.mean((p32 - p) ** 2) / np.mean(p**2))
-    print(f"[zero-friction] device: rms dx {dx:.2e} A, rel dp {dp:.2e}; "
-          f"float32-state oracle: rms dx {dx32:.2e} A, rel dp {dp32:.2e}", flush=True)
-    # Positions and momenta are float32 state (as in prolix's default dtype):
-    # the device must track float64 Verlet as closely as float32 storage allows.
-    assert dx < 4.0 * dx32 + 1e-6
-    assert dp < 4.0 * dp32 + 1e-6
+    eng = _engine(sysd)
+    eng.init(x0, Langevin(dt_fs=dt_fs, temperature=300.0, gamma_ps=0.0, seed=5), momenta=p0)
+    traj = [tuple(a.cpu().numpy().astype(np.float64) for a in eng.state())]
+    for _ in range(steps):
+        eng.steps(1)
+        traj.append(tuple(a.cpu().numpy().astype(np.float64) for a in eng.state()))
+    eng.close()
+    worst_dx = worst_dp = 0.0
+    for k in range(0, steps, 5):
+        x, p, _ = traj[k]
+        xv, pv, _ = verlet_step(x, p, ff(x))
+        xd, pd, _ = traj[k + 1]
+        worst_dx = max(worst_dx, float(np.max(np.abs(xd - xv))))
+        worst_dp = max(worst_dp, float(np.sqrt(np.mean((pd - pv) ** 2) / np.mean(pv**2))))
+    # whole trajectory against float64 Verlet from the same start
+    x, p, f = x0.copy(), p0.copy(), ff(x0)
+    for _ in range(steps):
+        x, p, f = verlet_step(x, p, f)
+    xg = traj[-1][0]
+    drift = float(np.sqrt(np.mean((xg - x) ** 2)))
+    ulp = float(np.max(np.spacing(np.abs(x0).astype(np.float32))))
+    print(f"[zero-friction] per step: max |dx| {worst_dx:.2e} A (float32 ulp {ulp:.1e}), rel dp {worst_dp:.2e}; "
+          f"{steps}-step rms dx {drift:.2e} A", flush=True)
+    assert worst_dx < 4.0 * ulp
+    assert worst_dp < 1e-5
+    assert drift < 1e-3
 
 
 def test_nve_drift_three_particles():

@@ -133,3 +133,19 @@ def test_serving_loop_small_trace():
     sysd = dict(reqs[0].system)
     terms = O.energy_terms(sysd, results[0].positions.astype(np.float64))
     assert e0[-1] == pytest.approx(terms["total"], rel=1e-5, abs=1e-3)
+
+
+def test_make_energy_fn_neighbor_argument_and_gb_cutoff():
+    """A neighbour list is accepted (its contents are not needed for the value),
+    and gb_cutoff=True gives the neighbour-list GB form of the golden vector."""
+    from prolix_amd.physics.system import make_energy_fn
+    from tests.golden import make_golden as G
+
+    sysd, cutoff, gbc, terms, forces, born = G.load("peptide138_gbcut12")
+    print("[gpu] make_energy_fn gb_cutoff", flush=True)
+    fn = make_energy_fn(None, sysd, implicit_solvent=True, cutoff=cutoff, gb_cutoff=True)
+    # dict systems carry no CMAP (PhysicsSystem.from_dict): compare without that term
+    expected = terms[-1] - terms[G.TERMS.index("cmap")]
+    nbrs = np.full((len(sysd["charges"]), 8), len(sysd["charges"]), np.int32)
+    assert fn(sysd["positions"], neighbor=nbrs) == pytest.approx(expected, rel=1e-6)
+    assert fn(sysd["positions"]) == fn(sysd["positions"], neighbor=nbrs)
