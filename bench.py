@@ -102,6 +102,9 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-large", action="store_true")
+    ap.add_argument("--calibration-copy-mb", type=float, default=0.0,
+                    help="also run one elementwise copy of this many MiB (float32), the reference kernel "
+                         "against which rocprofv3 FETCH_SIZE / WRITE_SIZE readings are scaled")
     ap.add_argument("--out", default=None)
     a = ap.parse_args()
     if a.gpus != 1:
@@ -181,6 +184,13 @@ def main():
     }
     log(f"passes (ms): {json.dumps({k: round(v, 4) for k, v in prof.items()})}")
     eng.close()
+    if a.calibration_copy_mb > 0:
+        src = torch.ones(int(a.calibration_copy_mb * 2**20) // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        torch.mul(src, 1.0, out=dst)  # reads and writes calibration_copy_mb MiB
+        torch.cuda.synchronize()
+        del src, dst
 
     # 5. the plain path: each system alone in its own engine, same parameters
     plain = None

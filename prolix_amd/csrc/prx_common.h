@@ -93,7 +93,7 @@ struct DeviceView {
   float4* pos;         // x, y, z, 0
   float4* mom;         // px, py, pz, 1/m
   float4* force;       // integrator force (float), w unused
-  float4* born4;       // rho, sr, sr^2, 0            (static)
+  float4* born4;       // rho, sr, sr^2, sr^3/2       (static)
   float4* nb4;         // q, sigma/2, sqrt(eps), 0    (static)
   double4* fin4;       // rho, R, q^2, ace_K          (static, FP64)
   float* sqrt_m;       // sqrt(mass)                  (static)

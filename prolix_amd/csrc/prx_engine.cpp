@@ -386,7 +386,7 @@ struct prx_engine {
       const double sig = std::max(d.sigmas[i], 1e-6);
       const double m = d.masses ? d.masses[i] : 1.0;
       if (!(m > 0.0)) throw std::invalid_argument("masses must be positive");
-      born4[i] = make_float4((float)rho, (float)sr, (float)(sr * sr), 0.0f);
+      born4[i] = make_float4((float)rho, (float)sr, (float)(sr * sr), (float)(0.5 * sr * sr * sr));
       nb4[i] = make_float4((float)q, (float)(0.5 * sig), (float)std::sqrt(std::max(d.epsilons[i], 0.0)), 0.0f);
       const double rn = (R - off) / 10.0 + 0.009;
       const double aceK = d.implicit_solvent

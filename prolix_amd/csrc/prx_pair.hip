@@ -1,10 +1,14 @@
 // All-pairs passes of the GB-OBC2 energy/force evaluation on gfx950.
 //
 // One wave64 per work item (a 64x64 tile pair I <= J of one system). Lane l
-// owns i atom I0 + l; the 64 j atoms of tile J live one per lane in registers
-// and rotate one lane per iteration with a DPP wave rotate, together with
-// their force accumulators, so every lane meets every j exactly once and no
-// two lanes touch the same j in one iteration. After the loop the wave stores
+// owns i atom I0 + l. The j tile's atom data is staged in the wave's own LDS
+// slice twice over (entries m and m + 64 hold j atom m), so at iteration k
+// lane l reads entry l + k -- j atom (l + k) mod 64 -- through an immediate
+// LDS offset; the j-side accumulators live in registers and rotate one lane
+// per iteration with a DPP wave rotate, so every lane meets every j exactly
+// once and no two lanes touch the same j in one iteration. (A DPP rotate
+// issues at about two thirds of the FMA rate on gfx950; the LDS reads do not
+// take VALU issue slots.) After the loop the wave stores
 // its i and j partial sums in the item's own slots of the partial buffer
 // (plain coalesced stores, no atomics); the tile gathers of prx_atom.hip add
 // them up in a fixed order, so results are bitwise reproducible.
@@ -48,6 +52,35 @@ __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x)
 __device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ float flog2(float x) { return __builtin_amdgcn_logf(x); }
 __device__ __forceinline__ float fexp2(float x) { return __builtin_amdgcn_exp2f(x); }
+
+// Orders this wave's LDS stores before its later LDS loads from other lanes.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+constexpr int kWavesPerBlock = 4;
+
+// Build-time tuning knobs (defaults are the measured best on MI355X):
+// unroll depth of the three pair loops, and where the j-side accumulators
+// live: 0 = registers rotated by DPP; 1 = LDS, entry (l + k) mod 64, added
+// with ds_add_f32 in iteration order (deterministic: one add per entry per
+// iteration, a wave's LDS operations complete in issue order).
+#ifndef PRX_UNROLL_BORN
+#define PRX_UNROLL_BORN 8
+#endif
+#ifndef PRX_UNROLL_FORCE
+#define PRX_UNROLL_FORCE 4
+#endif
+#ifndef PRX_UNROLL_CHAIN
+#define PRX_UNROLL_CHAIN 4
+#endif
+#ifndef PRX_LDS_ACC
+#define PRX_LDS_ACC 0
+#endif
+#define PRX_PRAGMA(x) _Pragma(#x)
+#define PRX_UNROLL(n) PRX_PRAGMA(unroll n)
 
 __device__ __forceinline__ int wave_id() {
   return __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)));
@@ -122,56 +155,79 @@ __device__ __forceinline__ bool obc_far(float r, float rho, float s) {
   return (r - s > rho) && (s <= PRX_OBC_FAR_XMAX * r);
 }
 
-__device__ __forceinline__ float obc_far_I(float s, float rinv) {
-  const float x = s * rinv, y = x * x;
+// Written with u = 1/r^2 and the per-atom constants s^2 and s^3/2 (born4.z, .w):
+//   I = (s^3/2) u^2 h(s^2 u),   dI/dr = -2 (s^3/2) u^2 r^-1 k(s^2 u),
+// so each direction costs one product for the argument, the Horner chain and
+// two products; u^2 and u^2/r are shared by both directions of a pair.
+__device__ __forceinline__ float obc_far_I(float s2, float s3h, float u, float u2) {
+  const float y = s2 * u;
   float h = kObcH[kObcHDeg];
 #pragma unroll
   for (int k = kObcHDeg - 1; k >= 0; --k) h = fmaf(h, y, kObcH[k]);
-  return 0.5f * x * y * rinv * h;
+  return (s3h * u2) * h;
 }
 
-__device__ __forceinline__ float obc_far_dI(float s, float rinv) {
-  const float x = s * rinv, y = x * x;
+__device__ __forceinline__ float obc_far_dI(float s2, float s3h, float u, float u2r) {
+  const float y = s2 * u;
   float q = kObcK[kObcKDeg];
 #pragma unroll
   for (int k = kObcKDeg - 1; k >= 0; --k) q = fmaf(q, y, kObcK[k]);
-  return -(x * y) * (rinv * rinv) * q;
+  return (-2.0f * s3h * u2r) * q;
 }
 
 // ---------------------------------------------------------------------------
 // Pass 1: Born sums
 // ---------------------------------------------------------------------------
 
+// LDS slice of one wave: j data staged twice (2 x 64 entries).
+struct BornLds {  // (x, y, z, rho), (s, s^2, s^3/2, 0); j accumulators
+  float4 a[128];
+  float4 b[128];
+#if PRX_LDS_ACC
+  float acc[64];
+#endif
+};
+
 template <bool GENERAL, bool DIAG>
 __device__ __forceinline__ void born_tile(const DeviceView& v, const PairItem& it, int lane, int w,
-                                          float cut2) {
+                                          float cut2, BornLds& L) {
   const int ni = item_ni(it.meta), nj = item_nj(it.meta);
   const int gi = it.i0 + min(lane, ni - 1);
   const int gj = it.j0 + min(lane, nj - 1);
   const float4 pi = v.pos[gi], bi = v.born4[gi];
-  float4 pj = v.pos[gj], bj = v.born4[gj];
-  float xj = pj.x, yj = pj.y, zj = pj.z, rhoj = bj.x, sj = bj.y, sj2 = bj.z;
+  {
+    const float4 pj = v.pos[gj], bj = v.born4[gj];
+    const float4 a = make_float4(pj.x, pj.y, pj.z, bj.x);
+    const float4 b = make_float4(bj.y, bj.z, bj.w, 0.0f);
+    L.a[lane] = a; L.a[lane + 64] = a;
+    L.b[lane] = b; L.b[lane + 64] = b;
+#if PRX_LDS_ACC
+    L.acc[lane] = 0.0f;
+#endif
+  }
+  wave_lds_sync();
   float accI = 0.0f, accJ = 0.0f;
   const bool ivalid = lane < ni;
-  int kbeg = 0, kend = 63;
-  if (DIAG) {
-    xj = rot(xj); yj = rot(yj); zj = rot(zj); rhoj = rot(rhoj); sj = rot(sj); sj2 = rot(sj2);
-    kbeg = 1; kend = 32;
-  }
+  constexpr int kbeg = DIAG ? 1 : 0, kend = DIAG ? 32 : 63;
+  PRX_UNROLL(PRX_UNROLL_BORN)
   for (int k = kbeg; k <= kend; ++k) {
+    const float4 ja = L.a[lane + k];
+    const float4 jb = L.b[lane + k];
+    const float xj = ja.x, yj = ja.y, zj = ja.z, rhoj = ja.w, sj = jb.x, sj2 = jb.y, sj3h = jb.z;
     const float dx = xj - pi.x, dy = yj - pi.y, dz = zj - pi.z;
     const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
     const float rinv = frsq(r2);
     const float r = r2 * rinv;
     const bool far1 = obc_far(r, bi.x, sj), far2 = obc_far(r, rhoj, bi.y);
+    const float u = rinv * rinv, u2 = u * u;
     BornPair bp;
     if (__all(far1 && far2)) {
-      bp.I1 = obc_far_I(sj, rinv);
-      bp.I2 = obc_far_I(bi.y, rinv);
+      bp.I1 = obc_far_I(sj2, sj3h, u, u2);
+      bp.I2 = obc_far_I(bi.z, bi.w, u, u2);
     } else {
       bp = born_pair(r, rinv, bi.x, bi.y, bi.z, rhoj, sj, sj2);
-      bp.I1 = far1 ? obc_far_I(sj, rinv) : bp.I1;
-      bp.I2 = far2 ? obc_far_I(bi.y, rinv) : bp.I2;
+      bp.I1 = far1 ? obc_far_I(sj2, sj3h, u, u2) : bp.I1;
+      bp.I2 = far2 ? obc_far_I(bi.z, bi.w, u, u2) : bp.I2;
     }
     if (GENERAL) {
       const int orig = (lane + k) & 63;
@@ -182,17 +238,22 @@ __device__ __forceinline__ void born_tile(const DeviceView& v, const PairItem& i
       bp.I2 = valid ? bp.I2 : 0.0f;
     }
     accI += bp.I1;
+#if PRX_LDS_ACC
+    atomicAdd(&L.acc[(lane + k) & 63], bp.I2);
+#else
     accJ += bp.I2;
-    if (k != kend) {
-      xj = rot(xj); yj = rot(yj); zj = rot(zj); rhoj = rot(rhoj); sj = rot(sj); sj2 = rot(sj2);
-      accJ = rot(accJ);
-    }
+    if (k != kend) accJ = rot(accJ);
+#endif
   }
-  const int nrot = DIAG ? 32 : 63;
-  const int home = (lane + nrot) & 63;
   float* pb = reinterpret_cast<float*>(v.part);  // scalar partials
   pb[part_slot(w, 0, lane)] = accI;
+#if PRX_LDS_ACC
+  wave_lds_sync();
+  pb[part_slot(w, 1, lane)] = L.acc[lane];
+#else
+  const int home = (lane + kend) & 63;
   pb[part_slot(w, 1, home)] = accJ;
+#endif
 }
 
 // Cutoff class of an item from its tiles' bounding boxes (wave-uniform).
@@ -215,6 +276,8 @@ __device__ __forceinline__ int classify(const DeviceView& v, const SlotConst& S,
 }
 
 __global__ __launch_bounds__(256) void k_born_pairs(DeviceView v) {
+  __shared__ BornLds lds[kWavesPerBlock];
+  BornLds& L = lds[threadIdx.x >> 6];
   const int w = wave_id();
   if (w >= v.n_items) return;
   const PairItem it = v.items[w];
@@ -230,11 +293,11 @@ __global__ __launch_bounds__(256) void k_born_pairs(DeviceView v) {
   if (S.gb_cut && cls == kClsFar) return;  // no partials: the gathers skip this item
   const float cut2 = (S.gb_cut && cls == kClsStraddle) ? S.cutoff2 : 0.0f;
   if (fl & kFlagDiag) {
-    born_tile<true, true>(v, it, lane, w, cut2);
+    born_tile<true, true>(v, it, lane, w, cut2, L);
   } else if (full && cut2 == 0.0f) {
-    born_tile<false, false>(v, it, lane, w, 0.0f);
+    born_tile<false, false>(v, it, lane, w, 0.0f, L);
   } else {
-    born_tile<true, false>(v, it, lane, w, cut2);
+    born_tile<true, false>(v, it, lane, w, cut2, L);
   }
 }
 
@@ -245,9 +308,17 @@ __global__ __launch_bounds__(256) void k_born_pairs(DeviceView v) {
 // LJC = false: GB only (items entirely beyond the LJ/Coulomb cutoff).
 // GENERAL tiles test r^2 < cut2 per pair when cut2 > 0, on LJ/Coulomb and, if
 // gb_cut, on GB too.
+struct PairLds {  // two float4 of j data per entry; j accumulators
+  float4 a[128];
+  float4 b[128];
+#if PRX_LDS_ACC
+  float4 acc[64];
+#endif
+};
+
 template <bool GENERAL, bool DIAG, bool ENERGY, bool LJC>
 __device__ __forceinline__ void force_tile(const DeviceView& v, const PairItem& it, int lane, int w, float cut2,
-                                           bool gb_cut) {
+                                           bool gb_cut, PairLds& L) {
   const int ni = item_ni(it.meta), nj = item_nj(it.meta);
   const SlotConst& S = v.slots[item_slot(it.meta)];
   const float kc = S.k_coulomb, gb2 = S.gb2;
@@ -257,8 +328,17 @@ __device__ __forceinline__ void force_tile(const DeviceView& v, const PairItem& 
   const int gi = it.i0 + min(lane, ni - 1);
   const int gj = it.j0 + min(lane, nj - 1);
   const float4 pi = v.pos[gi], nbi = v.nb4[gi], gbi = v.gb4[gi];
-  const float4 pj = v.pos[gj], nbj = v.nb4[gj], gbj = v.gb4[gj];
-  float xj = pj.x, yj = pj.y, zj = pj.z, qj = nbj.x, hsj = nbj.y, sej = nbj.z, Bj = gbj.x, aj = gbj.y;
+  {
+    const float4 pj = v.pos[gj], nbj = v.nb4[gj], gbj = v.gb4[gj];
+    const float4 a = make_float4(pj.x, pj.y, pj.z, nbj.x);    // x, y, z, q
+    const float4 b = make_float4(nbj.y, nbj.z, gbj.x, gbj.y);  // sigma/2, sqrt(eps), B, a
+    L.a[lane] = a; L.a[lane + 64] = a;
+    L.b[lane] = b; L.b[lane + 64] = b;
+#if PRX_LDS_ACC
+    L.acc[lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+#endif
+  }
+  wave_lds_sync();
   float fxi = 0.f, fyi = 0.f, fzi = 0.f, dbi = 0.f;
   float fxj = 0.f, fyj = 0.f, fzj = 0.f, dbj = 0.f;
   double egb = 0.0, ecl = 0.0, elj = 0.0;
@@ -268,13 +348,12 @@ __device__ __forceinline__ void force_tile(const DeviceView& v, const PairItem& 
     m1 = v.excl_masks[(size_t)it.excl * 128 + lane * 2 + 1];
   }
   const bool ivalid = lane < ni;
-  int kbeg = 0, kend = 63;
-  if (DIAG) {
-    xj = rot(xj); yj = rot(yj); zj = rot(zj); qj = rot(qj); hsj = rot(hsj); sej = rot(sej);
-    Bj = rot(Bj); aj = rot(aj);
-    kbeg = 1; kend = 32;
-  }
+  constexpr int kbeg = DIAG ? 1 : 0, kend = DIAG ? 32 : 63;
+  PRX_UNROLL(PRX_UNROLL_FORCE)
   for (int k = kbeg; k <= kend; ++k) {
+    const float4 ja = L.a[lane + k];
+    const float4 jb = L.b[lane + k];
+    const float xj = ja.x, yj = ja.y, zj = ja.z, qj = ja.w, hsj = jb.x, sej = jb.y, Bj = jb.z, aj = jb.w;
     const float dx = xj - pi.x, dy = yj - pi.y, dz = zj - pi.z;
     const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
     const float rinv = frsq(r2);
@@ -335,19 +414,31 @@ __device__ __forceinline__ void force_tile(const DeviceView& v, const PairItem& 
     }
     // F_i = -fr * (x_j - x_i)
     fxi = fmaf(-fr, dx, fxi); fyi = fmaf(-fr, dy, fyi); fzi = fmaf(-fr, dz, fzi);
-    fxj = fmaf(fr, dx, fxj); fyj = fmaf(fr, dy, fyj); fzj = fmaf(fr, dz, fzj);
     dbi = fmaf(g, Bj, dbi);
+#if PRX_LDS_ACC
+    {
+      float* ap = reinterpret_cast<float*>(&L.acc[(lane + k) & 63]);
+      atomicAdd(ap + 0, fr * dx);
+      atomicAdd(ap + 1, fr * dy);
+      atomicAdd(ap + 2, fr * dz);
+      atomicAdd(ap + 3, g * gbi.x);
+    }
+#else
+    fxj = fmaf(fr, dx, fxj); fyj = fmaf(fr, dy, fyj); fzj = fmaf(fr, dz, fzj);
     dbj = fmaf(g, gbi.x, dbj);
     if (k != kend) {
-      xj = rot(xj); yj = rot(yj); zj = rot(zj); qj = rot(qj); hsj = rot(hsj); sej = rot(sej);
-      Bj = rot(Bj); aj = rot(aj);
       fxj = rot(fxj); fyj = rot(fyj); fzj = rot(fzj); dbj = rot(dbj);
     }
+#endif
   }
-  const int nrot = DIAG ? 32 : 63;
-  const int home = (lane + nrot) & 63;
   v.part[part_slot(w, 0, lane)] = make_float4(fxi, fyi, fzi, dbi);
+#if PRX_LDS_ACC
+  wave_lds_sync();
+  v.part[part_slot(w, 1, lane)] = L.acc[lane];
+#else
+  const int home = (lane + kend) & 63;
   v.part[part_slot(w, 1, home)] = make_float4(fxj, fyj, fzj, dbj);
+#endif
   if (ENERGY) {
     for (int off = 32; off > 0; off >>= 1) {
       egb += __shfl_xor(egb, off);
@@ -364,6 +455,8 @@ __device__ __forceinline__ void force_tile(const DeviceView& v, const PairItem& 
 
 template <bool ENERGY>
 __global__ __launch_bounds__(256) void k_force_pairs(DeviceView v) {
+  __shared__ PairLds lds[kWavesPerBlock];
+  PairLds& L = lds[threadIdx.x >> 6];
   const int w = wave_id();
   if (w >= v.n_items) return;
   const PairItem it = v.items[w];
@@ -383,19 +476,19 @@ __global__ __launch_bounds__(256) void k_force_pairs(DeviceView v) {
     }
     // GB over every pair, nothing else (a far item is never diagonal)
     if (full)
-      force_tile<false, false, ENERGY, false>(v, it, lane, w, 0.0f, false);
+      force_tile<false, false, ENERGY, false>(v, it, lane, w, 0.0f, false, L);
     else
-      force_tile<true, false, ENERGY, false>(v, it, lane, w, 0.0f, false);
+      force_tile<true, false, ENERGY, false>(v, it, lane, w, 0.0f, false, L);
     return;
   }
   const float cut2 = cls == kClsStraddle ? S.cutoff2 : 0.0f;
   const bool gb_cut = S.gb_cut != 0;
   if (fl & kFlagDiag) {
-    force_tile<true, true, ENERGY, true>(v, it, lane, w, cut2, gb_cut);
+    force_tile<true, true, ENERGY, true>(v, it, lane, w, cut2, gb_cut, L);
   } else if (full && !(fl & kFlagExcl) && cut2 == 0.0f) {
-    force_tile<false, false, ENERGY, true>(v, it, lane, w, 0.0f, false);
+    force_tile<false, false, ENERGY, true>(v, it, lane, w, 0.0f, false, L);
   } else {
-    force_tile<true, false, ENERGY, true>(v, it, lane, w, cut2, gb_cut);
+    force_tile<true, false, ENERGY, true>(v, it, lane, w, cut2, gb_cut, L);
   }
 }
 
@@ -405,35 +498,45 @@ __global__ __launch_bounds__(256) void k_force_pairs(DeviceView v) {
 
 template <bool GENERAL, bool DIAG>
 __device__ __forceinline__ void chain_tile(const DeviceView& v, const PairItem& it, int lane, int w,
-                                           float cut2) {
+                                           float cut2, PairLds& L) {
   const int ni = item_ni(it.meta), nj = item_nj(it.meta);
   const int gi = it.i0 + min(lane, ni - 1);
   const int gj = it.j0 + min(lane, nj - 1);
   const float4 pi = v.pos[gi], bi = v.born4[gi];
   const float ci = v.gb4[gi].w;
-  const float4 pj = v.pos[gj], bj = v.born4[gj];
-  float xj = pj.x, yj = pj.y, zj = pj.z, rhoj = bj.x, sj = bj.y, sj2 = bj.z, cj = v.gb4[gj].w;
+  {
+    const float4 pj = v.pos[gj], bj = v.born4[gj];
+    const float4 a = make_float4(pj.x, pj.y, pj.z, bj.x);          // x, y, z, rho
+    const float4 b = make_float4(bj.y, bj.z, v.gb4[gj].w, bj.w);     // s, s^2, c, s^3/2
+    L.a[lane] = a; L.a[lane + 64] = a;
+    L.b[lane] = b; L.b[lane + 64] = b;
+#if PRX_LDS_ACC
+    L.acc[lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+#endif
+  }
+  wave_lds_sync();
   float fxi = 0.f, fyi = 0.f, fzi = 0.f, fxj = 0.f, fyj = 0.f, fzj = 0.f;
   const bool ivalid = lane < ni;
-  int kbeg = 0, kend = 63;
-  if (DIAG) {
-    xj = rot(xj); yj = rot(yj); zj = rot(zj); rhoj = rot(rhoj); sj = rot(sj); sj2 = rot(sj2); cj = rot(cj);
-    kbeg = 1; kend = 32;
-  }
+  constexpr int kbeg = DIAG ? 1 : 0, kend = DIAG ? 32 : 63;
+  PRX_UNROLL(PRX_UNROLL_CHAIN)
   for (int k = kbeg; k <= kend; ++k) {
+    const float4 ja = L.a[lane + k];
+    const float4 jb = L.b[lane + k];
+    const float xj = ja.x, yj = ja.y, zj = ja.z, rhoj = ja.w, sj = jb.x, sj2 = jb.y, cj = jb.z, sj3h = jb.w;
     const float dx = xj - pi.x, dy = yj - pi.y, dz = zj - pi.z;
     const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
     const float rinv = frsq(r2);
     const float r = r2 * rinv;
     const bool far1 = obc_far(r, bi.x, sj), far2 = obc_far(r, rhoj, bi.y);
     float d1, d2;  // dI(r; rho_i, s_j)/dr and dI(r; rho_j, s_i)/dr
+    const float u = rinv * rinv, u2r = u * u * rinv;
     if (__all(far1 && far2)) {
-      d1 = obc_far_dI(sj, rinv);
-      d2 = obc_far_dI(bi.y, rinv);
+      d1 = obc_far_dI(sj2, sj3h, u, u2r);
+      d2 = obc_far_dI(bi.z, bi.w, u, u2r);
     } else {
       const float rs = fminf(rinv, 1e4f);
-      d1 = far1 ? obc_far_dI(sj, rinv) : born_dr(r, rs, bi.x, sj, sj2);
-      d2 = far2 ? obc_far_dI(bi.y, rinv) : born_dr(r, rs, rhoj, bi.y, bi.z);
+      d1 = far1 ? obc_far_dI(sj2, sj3h, u, u2r) : born_dr(r, rs, bi.x, sj, sj2);
+      d2 = far2 ? obc_far_dI(bi.z, bi.w, u, u2r) : born_dr(r, rs, rhoj, bi.y, bi.z);
     }
     float gsc = fmaf(ci, d1, cj * d2) * rinv;           // F_i = gsc * (x_j - x_i)
     if (GENERAL) {
@@ -444,20 +547,33 @@ __device__ __forceinline__ void chain_tile(const DeviceView& v, const PairItem& 
       gsc = valid ? gsc : 0.0f;
     }
     fxi = fmaf(gsc, dx, fxi); fyi = fmaf(gsc, dy, fyi); fzi = fmaf(gsc, dz, fzi);
+#if PRX_LDS_ACC
+    {
+      float* ap = reinterpret_cast<float*>(&L.acc[(lane + k) & 63]);
+      atomicAdd(ap + 0, -gsc * dx);
+      atomicAdd(ap + 1, -gsc * dy);
+      atomicAdd(ap + 2, -gsc * dz);
+    }
+#else
     fxj = fmaf(-gsc, dx, fxj); fyj = fmaf(-gsc, dy, fyj); fzj = fmaf(-gsc, dz, fzj);
     if (k != kend) {
-      xj = rot(xj); yj = rot(yj); zj = rot(zj); rhoj = rot(rhoj); sj = rot(sj); sj2 = rot(sj2);
-      cj = rot(cj);
       fxj = rot(fxj); fyj = rot(fyj); fzj = rot(fzj);
     }
+#endif
   }
-  const int nrot = DIAG ? 32 : 63;
-  const int home = (lane + nrot) & 63;
   v.part[part_slot(w, 0, lane)] = make_float4(fxi, fyi, fzi, 0.0f);
+#if PRX_LDS_ACC
+  wave_lds_sync();
+  v.part[part_slot(w, 1, lane)] = L.acc[lane];
+#else
+  const int home = (lane + kend) & 63;
   v.part[part_slot(w, 1, home)] = make_float4(fxj, fyj, fzj, 0.0f);
+#endif
 }
 
 __global__ __launch_bounds__(256) void k_chain_pairs(DeviceView v) {
+  __shared__ PairLds lds[kWavesPerBlock];
+  PairLds& L = lds[threadIdx.x >> 6];
   const int w = wave_id();
   if (w >= v.n_items) return;
   const PairItem it = v.items[w];
@@ -472,11 +588,11 @@ __global__ __launch_bounds__(256) void k_chain_pairs(DeviceView v) {
     if (cls == kClsStraddle) cut2 = S.cutoff2;
   }
   if (fl & kFlagDiag) {
-    chain_tile<true, true>(v, it, lane, w, cut2);
+    chain_tile<true, true>(v, it, lane, w, cut2, L);
   } else if (full && cut2 == 0.0f) {
-    chain_tile<false, false>(v, it, lane, w, 0.0f);
+    chain_tile<false, false>(v, it, lane, w, 0.0f, L);
   } else {
-    chain_tile<true, false>(v, it, lane, w, cut2);
+    chain_tile<true, false>(v, it, lane, w, cut2, L);
   }
 }
 
