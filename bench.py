@@ -38,8 +38,7 @@ sys.path.insert(0, ROOT)
 METRIC = "aggregate ns/day + pair-interactions/s, batched GB-OBC2 MD; p50/p99 latency"
 FLOPS_PER_PAIR = {"born_pairs": 56, "force_pairs": 73, "chain_pairs": 62}  # algorithmic, per unordered pair
 FLOPS_TOTAL = 191
-PEAK_FP32 = 157.3  # TFLOP/s, MI355X dense FP32 (packed-FP32 rate)
-PEAK_FP32_UNPACKED = 78.6
+PEAK_FP32 = 157.3  # TFLOP/s, MI355X FP32 vector peak
 DT_FS = 1.0
 
 
@@ -166,6 +165,14 @@ def main():
     t_k = prof[pair_pass] * 1e-3
     achieved = FLOPS_PER_PAIR[pair_pass] * pairs_per_step / t_k / 1e12
     pair_total_s = sum(prof[k] for k in FLOPS_PER_PAIR) * 1e-3
+    full_name = {"born_pairs": "prx::k_born_pairs(prx::DeviceView)",
+                 "force_pairs": "void prx::k_force_pairs<false>(prx::DeviceView)",
+                 "chain_pairs": "prx::k_chain_pairs(prx::DeviceView)"}[pair_pass]
+    traffic = None
+    traffic_file = os.path.join(ROOT, "profiles", "hbm_traffic_batch256.json")
+    if os.path.exists(traffic_file) and a.n_systems == 256:
+        with open(traffic_file) as fh:
+            traffic = json.load(fh)["kernels"].get(full_name, {}).get("bytes")
     roofline = {
         "bound": "valu",
         "kernel": {"born_pairs": "k_born_pairs", "force_pairs": "k_force_pairs<false>",
@@ -174,8 +181,9 @@ def main():
         "peak": PEAK_FP32,
         "unit": "TFLOP/s",
         "frac": achieved / PEAK_FP32,
-        "traffic": None,
-        "peak_unpacked_fp32": PEAK_FP32_UNPACKED,
+        "traffic": traffic,
+        "traffic_unit": "bytes per launch (HBM, rocprofv3 FETCH_SIZE + WRITE_SIZE scaled by a 256 MiB copy; "
+                        "profiles/hbm_traffic_batch256.md)",
         "flops_per_pair": FLOPS_PER_PAIR[pair_pass],
         "pairs_per_launch": pairs_per_step,
         "kernel_ms": prof[pair_pass],
