@@ -108,12 +108,12 @@ __device__ __forceinline__ void for_tile_items(const DeviceView& v, const TileCt
   }
   for (int g0 = 0; g0 < n_row; g0 += 64) {
     const int n = g0 + c.lane;
-    const bool act = n < n_row && v.item_cls[r0 + c.q + kGW * n] != kClsFar;
+    const bool act = n < n_row && (v.item_cls[r0 + c.q + kGW * n] & kClsCutMask) != kClsFar;
     for (unsigned long long m = __ballot(act); m; m &= m - 1) add(r0 + c.q + kGW * (g0 + __builtin_ctzll(m)), 0);
   }
   for (int g0 = 0; g0 < n_col; g0 += 64) {
     const int n = g0 + c.lane;
-    const bool act = n < n_col && v.item_cls[col_item(min(n, n_col - 1))] != kClsFar;
+    const bool act = n < n_col && (v.item_cls[col_item(min(n, n_col - 1))] & kClsCutMask) != kClsFar;
     for (unsigned long long m = __ballot(act); m; m &= m - 1) add(col_item(g0 + __builtin_ctzll(m)), 1);
   }
 }
@@ -470,14 +470,16 @@ __device__ __forceinline__ int gather_wave() {
   return __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)));
 }
 
-// Bounding box of each tile's atoms at the current positions, one wave per tile.
+// Bounding box of each tile's atoms at the current positions, with the
+// tile's largest offset radius rho and descreening radius s; one wave per tile.
 __global__ __launch_bounds__(256) void k_tile_boxes(DeviceView v) {
   const int t = gather_wave();
   if (t >= v.n_tiles) return;
   const int4 tl = v.tiles[t];
   const int lane = threadIdx.x & 63;
-  const float4 p = v.pos[tl.x + min(lane, tl.y - 1)];
-  float lx = p.x, ly = p.y, lz = p.z, hx = p.x, hy = p.y, hz = p.z;
+  const int g = tl.x + min(lane, tl.y - 1);
+  const float4 p = v.pos[g], b = v.born4[g];
+  float lx = p.x, ly = p.y, lz = p.z, hx = p.x, hy = p.y, hz = p.z, mr = b.x, ms = b.y;
   for (int off = 32; off > 0; off >>= 1) {
     lx = fminf(lx, __shfl_xor(lx, off));
     ly = fminf(ly, __shfl_xor(ly, off));
@@ -485,9 +487,11 @@ __global__ __launch_bounds__(256) void k_tile_boxes(DeviceView v) {
     hx = fmaxf(hx, __shfl_xor(hx, off));
     hy = fmaxf(hy, __shfl_xor(hy, off));
     hz = fmaxf(hz, __shfl_xor(hz, off));
+    mr = fmaxf(mr, __shfl_xor(mr, off));
+    ms = fmaxf(ms, __shfl_xor(ms, off));
   }
-  if (lane == 0) v.tbox[2 * t] = make_float4(lx, ly, lz, 0.0f);
-  if (lane == 1) v.tbox[2 * t + 1] = make_float4(hx, hy, hz, 0.0f);
+  if (lane == 0) v.tbox[2 * t] = make_float4(lx, ly, lz, mr);
+  if (lane == 1) v.tbox[2 * t + 1] = make_float4(hx, hy, hz, ms);
 }
 
 void launch_tile_boxes(const DeviceView& v, hipStream_t s) {

@@ -32,12 +32,17 @@ struct PairItem {
 constexpr int kFlagDiag = 1;
 constexpr int kFlagExcl = 2;
 
-// Cutoff class of a work item, decided each evaluation from the bounding
-// boxes of its two tiles (a margin of 1e-5 in r^2 keeps the decision
-// conservative against float rounding of the per-pair test).
+// Class of a work item, decided each evaluation from the bounding boxes of
+// its two tiles. Bits 0-1: cutoff class (a margin of 1e-5 in r^2 keeps the
+// decision conservative against float rounding of the per-pair test).
 constexpr int kClsNear = 0;      // every pair closer than the cutoff (or no cutoff)
 constexpr int kClsStraddle = 1;  // per-pair r^2 < cutoff^2 test needed
 constexpr int kClsFar = 2;       // every pair at or beyond the cutoff
+constexpr int kClsCutMask = 3;
+// Bit 2: every pair of the item is in the OBC2 far field in both directions
+// (r - s_j > rho_i, s_j <= 0.6 r and the same with i, j swapped), so the Born
+// and chain passes use the far-field polynomials without a per-pair test.
+constexpr int kClsObcFar = 4;
 
 __host__ __device__ inline int32_t item_meta(int ni, int nj, int flags, int slot) {
   return ni | (nj << 7) | (flags << 14) | (slot << 16);
@@ -110,9 +115,8 @@ struct DeviceView {
   int32_t n_tiles;
   const unsigned long long* excl_masks;  // 128 per exclusion tile: lane * 2 + {bit0, bit1}
   double* e_items;     // 3 per item: GB pairs, Coulomb, LJ
-  float4* tbox;        // per tile t: bounding box lo (2t), hi (2t + 1) of its atoms
-  int32_t* item_cls;   // per item: cutoff class (systems with a cutoff only)
-  int32_t any_cut;     // 1 if some system has a cutoff (tile boxes are computed)
+  float4* tbox;        // per tile t: (lo.xyz, max rho) at 2t, (hi.xyz, max s) at 2t + 1
+  int32_t* item_cls;   // per item: class bits (kCls*), written by the Born pass
   // bonded
   BondEntry* bonds; int32_t bond_hw;
   AngleEntry* angles; int32_t angle_hw;
@@ -134,7 +138,7 @@ struct DeviceView {
 };
 
 // Launchers (defined in the .hip translation units).
-void launch_tile_boxes(const DeviceView& v, hipStream_t s);                // -> tbox
+void launch_tile_boxes(const DeviceView& v, hipStream_t s);                // -> tbox (every evaluation)
 void launch_born_pairs(const DeviceView& v, hipStream_t s);                // also -> item_cls
 void launch_born_gather(const DeviceView& v, hipStream_t s);               // -> B, dB/dI
 void launch_force_pairs(const DeviceView& v, bool energy, hipStream_t s);

@@ -580,13 +580,11 @@ struct prx_engine {
     std::vector<int32_t> map;
     std::vector<int4> tiles;
     offsets.assign(1, 0);
-    int32_t any_cut = 0;
     for (int32_t sid : order) {
       Slot& s = slots[sid];
       const int32_t ib = (int32_t)items.size();
       s.c.item_base = ib;
       s.c.tile0 = (int32_t)tiles.size();
-      if (s.c.cutoff2 > 0.0f) any_cut = 1;
       push_const(sid, s.c);
       const int nT = (s.n + prx::kTile - 1) / prx::kTile;
       for (int T = 0; T < nT; ++T)
@@ -633,7 +631,6 @@ struct prx_engine {
     v.n_tiles = (int32_t)tiles.size();
     v.tbox = d_tbox;
     v.item_cls = d_item_cls;
-    v.any_cut = any_cut;
     v.atom_hw = (int32_t)a_atoms.high_water();
     v.bond_hw = (int32_t)a_bond.high_water();
     v.angle_hw = (int32_t)a_ang.high_water();
@@ -649,7 +646,7 @@ struct prx_engine {
   // ---- evaluation -------------------------------------------------------
   // Everything of one force evaluation except the final chain gather.
   void force_passes(bool energy, hipStream_t s) {
-    if (v.any_cut) prx::launch_tile_boxes(v, s);
+    prx::launch_tile_boxes(v, s);
     prx::launch_born_pairs(v, s);
     prx::launch_born_gather(v, s);
     prx::launch_force_pairs(v, energy, s);
@@ -1079,7 +1076,7 @@ int prx_profile_passes(prx_engine* eng, int32_t n_reps, double* ms) {
         HIPCHECK(hipEventElapsedTime(&t, ev[k][0], ev[k][1]));
         ms[k] += t;
       };
-      if (eng->v.any_cut) prx::launch_tile_boxes(eng->v, s);
+      prx::launch_tile_boxes(eng->v, s);
       timed(0, [&] { prx::launch_born_pairs(eng->v, s); });
       timed(1, [&] { prx::launch_born_gather(eng->v, s); });
       timed(2, [&] { prx::launch_force_pairs(eng->v, false, s); });

@@ -180,7 +180,7 @@ __device__ __forceinline__ float obc_far_dI(float s2, float s3h, float u, float 
 // ---------------------------------------------------------------------------
 
 // LDS slice of one wave: j data staged twice (2 x 64 entries).
-struct BornLds {  // (x, y, z, rho), (s, s^2, s^3/2, 0); j accumulators
+struct BornLds {  // (x, y, z, s^2), (s^3/2, rho, s, 0); j accumulators
   float4 a[128];
   float4 b[128];
 #if PRX_LDS_ACC
@@ -188,7 +188,8 @@ struct BornLds {  // (x, y, z, rho), (s, s^2, s^3/2, 0); j accumulators
 #endif
 };
 
-template <bool GENERAL, bool DIAG>
+// FAR: every pair of the item is in the far field (no per-pair test).
+template <bool GENERAL, bool DIAG, bool FAR>
 __device__ __forceinline__ void born_tile(const DeviceView& v, const PairItem& it, int lane, int w,
                                           float cut2, BornLds& L) {
   const int ni = item_ni(it.meta), nj = item_nj(it.meta);
@@ -197,8 +198,9 @@ __device__ __forceinline__ void born_tile(const DeviceView& v, const PairItem& i
   const float4 pi = v.pos[gi], bi = v.born4[gi];
   {
     const float4 pj = v.pos[gj], bj = v.born4[gj];
-    const float4 a = make_float4(pj.x, pj.y, pj.z, bj.x);
-    const float4 b = make_float4(bj.y, bj.z, bj.w, 0.0f);
+    // the far-field loop reads only a and b.x
+    const float4 a = make_float4(pj.x, pj.y, pj.z, bj.z);
+    const float4 b = make_float4(bj.w, bj.x, bj.y, 0.0f);
     L.a[lane] = a; L.a[lane + 64] = a;
     L.b[lane] = b; L.b[lane + 64] = b;
 #if PRX_LDS_ACC
@@ -212,22 +214,27 @@ __device__ __forceinline__ void born_tile(const DeviceView& v, const PairItem& i
   PRX_UNROLL(PRX_UNROLL_BORN)
   for (int k = kbeg; k <= kend; ++k) {
     const float4 ja = L.a[lane + k];
-    const float4 jb = L.b[lane + k];
-    const float xj = ja.x, yj = ja.y, zj = ja.z, rhoj = ja.w, sj = jb.x, sj2 = jb.y, sj3h = jb.z;
+    const float xj = ja.x, yj = ja.y, zj = ja.z, sj2 = ja.w, sj3h = L.b[lane + k].x;
     const float dx = xj - pi.x, dy = yj - pi.y, dz = zj - pi.z;
     const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
     const float rinv = frsq(r2);
-    const float r = r2 * rinv;
-    const bool far1 = obc_far(r, bi.x, sj), far2 = obc_far(r, rhoj, bi.y);
     const float u = rinv * rinv, u2 = u * u;
     BornPair bp;
-    if (__all(far1 && far2)) {
+    if (FAR) {
       bp.I1 = obc_far_I(sj2, sj3h, u, u2);
       bp.I2 = obc_far_I(bi.z, bi.w, u, u2);
     } else {
-      bp = born_pair(r, rinv, bi.x, bi.y, bi.z, rhoj, sj, sj2);
-      bp.I1 = far1 ? obc_far_I(sj2, sj3h, u, u2) : bp.I1;
-      bp.I2 = far2 ? obc_far_I(bi.z, bi.w, u, u2) : bp.I2;
+      const float rhoj = L.b[lane + k].y, sj = L.b[lane + k].z;
+      const float r = r2 * rinv;
+      const bool far1 = obc_far(r, bi.x, sj), far2 = obc_far(r, rhoj, bi.y);
+      if (__all(far1 && far2)) {
+        bp.I1 = obc_far_I(sj2, sj3h, u, u2);
+        bp.I2 = obc_far_I(bi.z, bi.w, u, u2);
+      } else {
+        bp = born_pair(r, rinv, bi.x, bi.y, bi.z, rhoj, sj, sj2);
+        bp.I1 = far1 ? obc_far_I(sj2, sj3h, u, u2) : bp.I1;
+        bp.I2 = far2 ? obc_far_I(bi.z, bi.w, u, u2) : bp.I2;
+      }
     }
     if (GENERAL) {
       const int orig = (lane + k) & 63;
@@ -256,7 +263,10 @@ __device__ __forceinline__ void born_tile(const DeviceView& v, const PairItem& i
 #endif
 }
 
-// Cutoff class of an item from its tiles' bounding boxes (wave-uniform).
+// Class bits of an item from its tiles' bounding boxes (wave-uniform). The
+// OBC2 far-field bit needs the box distance above every pair's threshold,
+// max(s_j + rho_i, s_i + rho_j, s / 0.6) over the two tiles, with a 1e-4
+// relative margin on the squared distance.
 __device__ __forceinline__ int classify(const DeviceView& v, const SlotConst& S, const PairItem& it) {
   const int tI = S.tile0 + ((it.i0 - S.a0) >> 6);
   const int tJ = S.tile0 + ((it.j0 - S.a0) >> 6);
@@ -270,9 +280,15 @@ __device__ __forceinline__ int classify(const DeviceView& v, const SlotConst& S,
   const float ez = fmaxf(hj.z - li.z, hi.z - lj.z);
   const float dmin2 = gx * gx + gy * gy + gz * gz;
   const float dmax2 = ex * ex + ey * ey + ez * ez;
-  if (dmin2 >= S.cutoff2 * 1.00001f) return kClsFar;
-  if (dmax2 < S.cutoff2 * 0.99999f) return kClsNear;
-  return kClsStraddle;
+  const float thr = fmaxf(fmaxf(hj.w + li.w, hi.w + lj.w), fmaxf(hi.w, hj.w) * (1.0f / PRX_OBC_FAR_XMAX));
+  int cls = (tI != tJ && dmin2 > thr * thr * 1.0001f) ? kClsObcFar : 0;
+  if (S.cutoff2 > 0.0f) {
+    if (dmin2 >= S.cutoff2 * 1.00001f)
+      cls |= kClsFar;
+    else if (!(dmax2 < S.cutoff2 * 0.99999f))
+      cls |= kClsStraddle;
+  }
+  return cls;
 }
 
 __global__ __launch_bounds__(256) void k_born_pairs(DeviceView v) {
@@ -285,19 +301,22 @@ __global__ __launch_bounds__(256) void k_born_pairs(DeviceView v) {
   const int fl = item_flags(it.meta);
   const bool full = item_ni(it.meta) == kTile && item_nj(it.meta) == kTile;
   const SlotConst& S = v.slots[item_slot(it.meta)];
-  int cls = kClsNear;
-  if (S.cutoff2 > 0.0f) {
-    cls = classify(v, S, it);
-    if (lane == 0) v.item_cls[w] = cls;
-  }
-  if (S.gb_cut && cls == kClsFar) return;  // no partials: the gathers skip this item
-  const float cut2 = (S.gb_cut && cls == kClsStraddle) ? S.cutoff2 : 0.0f;
+  const int cls = classify(v, S, it);
+  if (lane == 0) v.item_cls[w] = cls;
+  const int ccut = cls & kClsCutMask;
+  if (S.gb_cut && ccut == kClsFar) return;  // no partials: the gathers skip this item
+  const float cut2 = (S.gb_cut && ccut == kClsStraddle) ? S.cutoff2 : 0.0f;
   if (fl & kFlagDiag) {
-    born_tile<true, true>(v, it, lane, w, cut2, L);
+    born_tile<true, true, false>(v, it, lane, w, cut2, L);
+  } else if (cls & kClsObcFar) {
+    if (full && cut2 == 0.0f)
+      born_tile<false, false, true>(v, it, lane, w, 0.0f, L);
+    else
+      born_tile<true, false, true>(v, it, lane, w, cut2, L);
   } else if (full && cut2 == 0.0f) {
-    born_tile<false, false>(v, it, lane, w, 0.0f, L);
+    born_tile<false, false, false>(v, it, lane, w, 0.0f, L);
   } else {
-    born_tile<true, false>(v, it, lane, w, cut2, L);
+    born_tile<true, false, false>(v, it, lane, w, cut2, L);
   }
 }
 
@@ -328,6 +347,7 @@ __device__ __forceinline__ void force_tile(const DeviceView& v, const PairItem& 
   const int gi = it.i0 + min(lane, ni - 1);
   const int gj = it.j0 + min(lane, nj - 1);
   const float4 pi = v.pos[gi], nbi = v.nb4[gi], gbi = v.gb4[gi];
+  const float q_kc = kc * nbi.x, q_gb = gb2 * nbi.x;
   {
     const float4 pj = v.pos[gj], nbj = v.nb4[gj], gbj = v.gb4[gj];
     const float4 a = make_float4(pj.x, pj.y, pj.z, nbj.x);    // x, y, z, q
@@ -358,11 +378,10 @@ __device__ __forceinline__ void force_tile(const DeviceView& v, const PairItem& 
     const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
     const float rinv = frsq(r2);
     const float rinv2 = rinv * rinv;
-    const float qq = nbi.x * qj;
-    // Coulomb + LJ
+    // Coulomb + LJ (q_kc = k_coulomb q_i, q_gb = gb2 q_i are per-lane constants)
     float ec = 0.0f, el = 0.0f, fl = 0.0f;
     if (LJC) {
-      ec = kc * qq * rinv;
+      ec = (q_kc * qj) * rinv;
       const float sg = nbi.y + hsj;
       const float ep = nbi.z * sej;
       const float s2 = sg * sg * rinv2;
@@ -376,10 +395,11 @@ __device__ __forceinline__ void force_tile(const DeviceView& v, const PairItem& 
     const float ex = fexp2(-x);
     const float f2 = fmaf(BB, ex, r2);
     const float finv = frsq(f2);
-    const float eg = gb2 * qq * finv;
-    const float t = eg * finv * finv;                 // gb2 qq / f^3
-    float fgb = t * fmaf(-0.25f, ex, 1.0f);            // -dE/dr / r
-    float g = -0.5f * t * ex * fmaf(x, kLn2, 1.0f);    // dE/dB_i = g B_j, dE/dB_j = g B_i
+    const float eg = (q_gb * qj) * finv;
+    const float t = eg * (finv * finv);                // gb2 qq / f^3
+    const float te = t * ex;
+    float fgb = fmaf(-0.25f, te, t);                   // -dE/dr / r
+    float g = te * fmaf(x, kLn2, 1.0f);                // dE/dB_i = -g B_j / 2, dE/dB_j = -g B_i / 2
     if (GENERAL) {
       const int orig = (lane + k) & 63;
       bool valid = ivalid && orig < nj;
@@ -421,7 +441,7 @@ __device__ __forceinline__ void force_tile(const DeviceView& v, const PairItem& 
       atomicAdd(ap + 0, fr * dx);
       atomicAdd(ap + 1, fr * dy);
       atomicAdd(ap + 2, fr * dz);
-      atomicAdd(ap + 3, g * gbi.x);
+      atomicAdd(ap + 3, -0.5f * g * gbi.x);
     }
 #else
     fxj = fmaf(fr, dx, fxj); fyj = fmaf(fr, dy, fyj); fzj = fmaf(fr, dz, fzj);
@@ -431,13 +451,13 @@ __device__ __forceinline__ void force_tile(const DeviceView& v, const PairItem& 
     }
 #endif
   }
-  v.part[part_slot(w, 0, lane)] = make_float4(fxi, fyi, fzi, dbi);
+  v.part[part_slot(w, 0, lane)] = make_float4(fxi, fyi, fzi, -0.5f * dbi);
 #if PRX_LDS_ACC
   wave_lds_sync();
   v.part[part_slot(w, 1, lane)] = L.acc[lane];
 #else
   const int home = (lane + kend) & 63;
-  v.part[part_slot(w, 1, home)] = make_float4(fxj, fyj, fzj, dbj);
+  v.part[part_slot(w, 1, home)] = make_float4(fxj, fyj, fzj, -0.5f * dbj);
 #endif
   if (ENERGY) {
     for (int off = 32; off > 0; off >>= 1) {
@@ -464,7 +484,7 @@ __global__ __launch_bounds__(256) void k_force_pairs(DeviceView v) {
   const int fl = item_flags(it.meta);
   const bool full = item_ni(it.meta) == kTile && item_nj(it.meta) == kTile;
   const SlotConst& S = v.slots[item_slot(it.meta)];
-  const int cls = S.cutoff2 > 0.0f ? v.item_cls[w] : kClsNear;
+  const int cls = S.cutoff2 > 0.0f ? (v.item_cls[w] & kClsCutMask) : kClsNear;
   if (cls == kClsFar) {
     if (S.gb_cut) {
       if (ENERGY && lane == 0) {
@@ -496,7 +516,7 @@ __global__ __launch_bounds__(256) void k_force_pairs(DeviceView v) {
 // Pass 3: chain rule through the Born radii
 // ---------------------------------------------------------------------------
 
-template <bool GENERAL, bool DIAG>
+template <bool GENERAL, bool DIAG, bool FAR>
 __device__ __forceinline__ void chain_tile(const DeviceView& v, const PairItem& it, int lane, int w,
                                            float cut2, PairLds& L) {
   const int ni = item_ni(it.meta), nj = item_nj(it.meta);
@@ -506,8 +526,8 @@ __device__ __forceinline__ void chain_tile(const DeviceView& v, const PairItem& 
   const float ci = v.gb4[gi].w;
   {
     const float4 pj = v.pos[gj], bj = v.born4[gj];
-    const float4 a = make_float4(pj.x, pj.y, pj.z, bj.x);          // x, y, z, rho
-    const float4 b = make_float4(bj.y, bj.z, v.gb4[gj].w, bj.w);     // s, s^2, c, s^3/2
+    const float4 a = make_float4(pj.x, pj.y, pj.z, bj.z);          // x, y, z, s^2
+    const float4 b = make_float4(bj.w, v.gb4[gj].w, bj.x, bj.y);     // s^3/2, c, rho, s
     L.a[lane] = a; L.a[lane + 64] = a;
     L.b[lane] = b; L.b[lane + 64] = b;
 #if PRX_LDS_ACC
@@ -521,16 +541,20 @@ __device__ __forceinline__ void chain_tile(const DeviceView& v, const PairItem& 
   PRX_UNROLL(PRX_UNROLL_CHAIN)
   for (int k = kbeg; k <= kend; ++k) {
     const float4 ja = L.a[lane + k];
-    const float4 jb = L.b[lane + k];
-    const float xj = ja.x, yj = ja.y, zj = ja.z, rhoj = ja.w, sj = jb.x, sj2 = jb.y, cj = jb.z, sj3h = jb.w;
+    const float2 jb = *reinterpret_cast<const float2*>(&L.b[lane + k]);  // the far field needs only these
+    const float xj = ja.x, yj = ja.y, zj = ja.z, sj2 = ja.w, sj3h = jb.x, cj = jb.y;
     const float dx = xj - pi.x, dy = yj - pi.y, dz = zj - pi.z;
     const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
     const float rinv = frsq(r2);
+    const float rhoj = L.b[lane + k].z, sj = L.b[lane + k].w;  // near field only
     const float r = r2 * rinv;
     const bool far1 = obc_far(r, bi.x, sj), far2 = obc_far(r, rhoj, bi.y);
     float d1, d2;  // dI(r; rho_i, s_j)/dr and dI(r; rho_j, s_i)/dr
     const float u = rinv * rinv, u2r = u * u * rinv;
-    if (__all(far1 && far2)) {
+    if (FAR) {
+      d1 = obc_far_dI(sj2, sj3h, u, u2r);
+      d2 = obc_far_dI(bi.z, bi.w, u, u2r);
+    } else if (__all(far1 && far2)) {
       d1 = obc_far_dI(sj2, sj3h, u, u2r);
       d2 = obc_far_dI(bi.z, bi.w, u, u2r);
     } else {
@@ -581,18 +605,23 @@ __global__ __launch_bounds__(256) void k_chain_pairs(DeviceView v) {
   const int fl = item_flags(it.meta);
   const bool full = item_ni(it.meta) == kTile && item_nj(it.meta) == kTile;
   const SlotConst& S = v.slots[item_slot(it.meta)];
+  const int cls = v.item_cls[w];
   float cut2 = 0.0f;
   if (S.gb_cut) {
-    const int cls = v.item_cls[w];
-    if (cls == kClsFar) return;
-    if (cls == kClsStraddle) cut2 = S.cutoff2;
+    if ((cls & kClsCutMask) == kClsFar) return;
+    if ((cls & kClsCutMask) == kClsStraddle) cut2 = S.cutoff2;
   }
   if (fl & kFlagDiag) {
-    chain_tile<true, true>(v, it, lane, w, cut2, L);
+    chain_tile<true, true, false>(v, it, lane, w, cut2, L);
+  } else if (cls & kClsObcFar) {
+    if (full && cut2 == 0.0f)
+      chain_tile<false, false, true>(v, it, lane, w, 0.0f, L);
+    else
+      chain_tile<true, false, true>(v, it, lane, w, cut2, L);
   } else if (full && cut2 == 0.0f) {
-    chain_tile<false, false>(v, it, lane, w, 0.0f, L);
+    chain_tile<false, false, false>(v, it, lane, w, 0.0f, L);
   } else {
-    chain_tile<true, false>(v, it, lane, w, cut2, L);
+    chain_tile<true, false, false>(v, it, lane, w, cut2, L);
   }
 }
 
