@@ -95,7 +95,7 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--n-systems", type=int, default=256)
-    ap.add_argument("--min-steps", type=int, default=200)
+    ap.add_argument("--min-steps", type=int, default=500)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=32)
@@ -301,7 +301,8 @@ def main():
         "plain_path": plain,
         "serving": serving,
         "large_cutoff": large_out,
-        "temperature_K": {"mean": float(np.mean(temp)), "min": float(np.min(temp)), "max": float(np.max(temp))},
+        "temperature_K": {"mean": float(np.mean(temp)), "median": float(np.median(temp)), "min": float(np.min(temp)),
+                          "max": float(np.max(temp)), "systems_above_1000K": int(np.sum(temp > 1000.0))},
         "safety_cap_events": {"vlimit": int(warn[0]), "force_cap": int(warn[1]), "dx_cap": int(warn[3])},
     }
     line = json.dumps(out)

@@ -729,3 +729,46 @@ def baoab_step(pos, mom, force, mass, dt, kT, gamma, noise, force_fn):
     force = force_fn(pos)
     mom = mom + 0.5 * dt * force
     return pos, mom, force
+
+
+# ----------------------------------------------------------------------------
+# FIRE minimisation (jax_md.minimize.fire_descent as staged by prolix's
+# simulate.py:615-748), float64, one system
+# ----------------------------------------------------------------------------
+
+def fire_minimize(force_fn, pos, mass, steps, dt_start, dt_max, force_cap=None, disp_cap=0.1, n_min=5,
+                  f_inc=1.1, f_dec=0.5, alpha_start=0.1, f_alpha=0.99):
+    """Returns (positions, momenta, forces, dt, alpha, n_pos) after ``steps`` steps."""
+    x = np.asarray(pos, np.float64).copy()
+    m = np.asarray(mass, np.float64).reshape(-1, 1)
+    p = np.zeros_like(x)
+    f = force_fn(x)
+    dt, alpha, n_pos = float(dt_start), float(alpha_start), 0
+    for _ in range(int(steps)):
+        x0 = x
+        if force_cap is not None:  # simulate.py:618-623
+            fn = np.linalg.norm(f, axis=1, keepdims=True)
+            f = f * np.minimum(1.0, force_cap / (fn + 1e-8))
+            f = np.where(np.isfinite(f), f, 0.0)
+        p = p + 0.5 * dt * f
+        x = x0 + dt * p / m
+        f = force_fn(x)
+        p = p + 0.5 * dt * f
+        f_norm = math.sqrt(float(np.sum(f * f)) + 1e-6)
+        p_norm = math.sqrt(float(np.sum(p * p)))
+        fdotp = float(np.sum(f * p))
+        p = p + alpha * (f * p_norm / f_norm - p)
+        n_pos = n_pos + 1 if fdotp >= 0 else 0
+        if fdotp > 0 and n_pos > n_min:
+            dt = min(dt * f_inc, dt_max)
+            alpha = alpha * f_alpha
+        if fdotp < 0:
+            dt = dt * f_dec
+            alpha = alpha_start
+            p = np.zeros_like(p)
+        if force_cap is not None and disp_cap is not None:  # simulate.py:628-638
+            dr = x - x0
+            sc = np.minimum(1.0, disp_cap / (np.linalg.norm(dr, axis=1, keepdims=True) + 1e-8))
+            x = x0 + dr * sc
+            p = p * sc
+    return x, p, f, dt, alpha, n_pos

@@ -669,12 +669,19 @@ struct prx_engine {
     for (auto& g : graphs)
       if (g.first == n_steps) return g.second;
     hipGraphExec_t exec = nullptr;
-    hipGraph_t g;
+    hipGraph_t g = nullptr;
     HIPCHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
-    prx::launch_step_begin(v, stream);
-    for (int k = 0; k < n_steps; ++k) {
-      force_passes(false, stream);
-      prx::launch_chain_gather(v, k + 1 == n_steps ? 2 : 1, stream);
+    try {
+      prx::launch_step_begin(v, stream);
+      for (int k = 0; k < n_steps; ++k) {
+        force_passes(false, stream);
+        prx::launch_chain_gather(v, k + 1 == n_steps ? 2 : 1, stream);
+      }
+      HIPCHECK(hipGetLastError());
+    } catch (...) {
+      // leave the stream out of capture mode before reporting the failure
+      if (hipStreamEndCapture(stream, &g) == hipSuccess && g) hipGraphDestroy(g);
+      throw;
     }
     HIPCHECK(hipStreamEndCapture(stream, &g));
     HIPCHECK(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));

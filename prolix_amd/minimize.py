@@ -45,3 +45,116 @@ def minimize(engine, positions, steps: int = 200, dt_max: float = 4e-3, max_disp
         accept = ce < best_e
         x = t.where(accept[sid, None], cx, best_x)
     return x
+
+
+# ----------------------------------------------------------------------------
+# FIRE (Bitzek et al., PRL 97, 170201 (2006)) in the form of
+# jax_md.minimize.fire_descent, which prolix's production path stages at
+# simulate.py:642-748: a velocity-Verlet step in momentum form with the
+# current dt, then P <- P + alpha (F |P| / |F| - P) with |F| regularised by
+# 1e-6, then the dt / alpha / n_pos update and a momentum reset when F.P < 0.
+# The norms and F.P run over one system's atoms, so every system of a batch
+# minimises independently. Units: A, kcal/mol, amu, and AKMA time for dt.
+# Optional caps as prolix's _make_capped_fire_apply (simulate.py:615-640):
+# per-atom force clipped to force_cap (non-finite -> 0) before the step, and
+# per-atom displacement clipped to disp_cap after it, with the momentum
+# scaled by the same factor.
+# ----------------------------------------------------------------------------
+
+FIRE_DEFAULTS = dict(n_min=5, f_inc=1.1, f_dec=0.5, alpha_start=0.1, f_alpha=0.99)
+
+
+class FireState:
+    """Batched FIRE state: positions, momenta, forces (device), per-system dt, alpha, n_pos."""
+
+    def __init__(self, x, p, f, dt, alpha, n_pos):
+        self.positions, self.momenta, self.forces = x, p, f
+        self.dt, self.alpha, self.n_pos = dt, alpha, n_pos
+
+
+def fire_init(engine, positions, dt_start, alpha_start=0.1):
+    t = engine.torch
+    x = engine._pos_tensor(positions).clone()
+    _, f, _ = engine.energy_forces(x, energies=False)
+    ns = engine.n_systems
+    dt = t.full((ns,), float(dt_start), dtype=t.float64, device=x.device)
+    alpha = t.full((ns,), float(alpha_start), dtype=t.float64, device=x.device)
+    n_pos = t.zeros((ns,), dtype=t.int64, device=x.device)
+    return FireState(x, t.zeros_like(x), f.clone(), dt, alpha, n_pos)
+
+
+def fire_step(engine, state, masses, dt_max, force_cap=None, disp_cap=0.1, **kw):
+    """One FIRE step of every system (masses: packed (N,) tensor in amu)."""
+    t = engine.torch
+    o = dict(FIRE_DEFAULTS, **kw)
+    sid = t.as_tensor(np.repeat(np.arange(engine.n_systems), engine.sizes), device=state.positions.device)
+    ns = engine.n_systems
+    m = masses[:, None]
+    x0, p, f = state.positions, state.momenta, state.forces
+    if force_cap is not None:
+        fn = t.linalg.vector_norm(f, dim=1, keepdim=True)
+        f = f * t.clamp(force_cap / (fn + 1e-8), max=1.0)
+        f = t.where(t.isfinite(f), f, t.zeros_like(f))
+    dt = state.dt[sid, None].to(x0.dtype)
+    p = p + 0.5 * dt * f
+    x = x0 + dt * p / m
+    _, f_new, _ = engine.energy_forces(x, energies=False)
+    f_new = f_new.clone()
+    p = p + 0.5 * dt * f_new
+
+    def per_system(v):
+        return t.zeros(ns, dtype=t.float64, device=v.device).index_add_(0, sid, v.to(t.float64))
+
+    f_norm = t.sqrt(per_system((f_new * f_new).sum(1)) + 1e-6)
+    p_norm = t.sqrt(per_system((p * p).sum(1)))
+    fdotp = per_system((f_new * p).sum(1))
+    alpha = state.alpha
+    scale = (p_norm / f_norm)[sid, None].to(p.dtype)
+    p = p + alpha[sid, None].to(p.dtype) * (f_new * scale - p)
+    n_pos = t.where(fdotp >= 0, state.n_pos + 1, t.zeros_like(state.n_pos))
+    dt_s, dtm = state.dt, t.full_like(state.dt, float(dt_max))
+    grow = (fdotp > 0) & (n_pos > o["n_min"])
+    dt_s = t.where(grow, t.minimum(dt_s * o["f_inc"], dtm), dt_s)
+    dt_s = t.where(fdotp < 0, dt_s * o["f_dec"], dt_s)
+    alpha = t.where(grow, alpha * o["f_alpha"], alpha)
+    alpha = t.where(fdotp < 0, t.full_like(alpha, o["alpha_start"]), alpha)
+    p = p * (fdotp >= 0)[sid, None].to(p.dtype)
+    if force_cap is not None and disp_cap is not None:
+        # as the reference, the state keeps the force of the uncapped positions
+        dr = x - x0
+        s = t.clamp(disp_cap / (t.linalg.vector_norm(dr, dim=1, keepdim=True) + 1e-8), max=1.0)
+        x = x0 + dr * s
+        p = p * s
+    return FireState(x, p, f_new, dt_s, alpha, n_pos)
+
+
+def fire(engine, positions, steps, masses, dt_start, dt_max, force_cap=None, disp_cap=0.1, state=None, **kw):
+    """Run ``steps`` FIRE steps; returns the final FireState."""
+    t = engine.torch
+    mt = t.as_tensor(np.asarray(masses, np.float64).reshape(-1), dtype=t.float32, device=engine.device)
+    st = state if state is not None else fire_init(engine, positions, dt_start, kw.get("alpha_start", 0.1))
+    for _ in range(int(steps)):
+        st = fire_step(engine, st, mt, dt_max, force_cap, disp_cap, **kw)
+    return st
+
+
+# prolix's staged FIRE (simulate.py:668-748) without its soft-core stages: the
+# engine evaluates the standard energy, so each stage keeps the reference's
+# force cap and length on the full potential.
+FIRE_STAGES = ((10.0, 500), (10.0, 500), (100.0, 500), (1000.0, 1000), (None, 3000))
+
+
+def staged_fire(engine, positions, masses, stages=FIRE_STAGES):
+    """Dynamic dt as simulate.py:642-656, then the capped stages; returns packed positions."""
+    t = engine.torch
+    _, f0, _ = engine.energy_forces(positions, energies=False)
+    max_grad = float(t.linalg.vector_norm(f0, dim=1).max())
+    dt_start = float(np.clip(0.001 / (max_grad + 1e-8), 1e-7, 0.001))
+    dt_max = min(dt_start * 10.0, 0.01)
+    x = engine._pos_tensor(positions)
+    for cap, n in stages:
+        st = fire(engine, x, n, masses, dt_start, dt_max, force_cap=cap)
+        x = st.positions
+        dt_start = min(dt_start * 2.0, 0.001)
+        dt_max = min(dt_start * 10.0, 0.01)
+    return x

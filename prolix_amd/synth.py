@@ -257,6 +257,13 @@ def _ca_walk(n_res, rng, radius):
     return ca
 
 
+# Equilibrium bond angles are kept in [80, 150] degrees: a torsion's gradient
+# diverges as either of its bond angles reaches 180 degrees, and an equilibrium
+# angle near 180 (which free placement sometimes produces) puts thermal
+# motion on that singularity.
+ANGLE_MIN_DEG, ANGLE_MAX_DEG = 80.0, 150.0
+
+
 def _place_child(pos, grid, placed_mask, p_idx, g_pos, bond, angle_deg, sig, idx, skip, rng, ntry=24,
                  close13=()):
     """Place atom ``idx`` bonded to ``p_idx`` at ``bond`` A, angle to g_pos ~ angle_deg."""
@@ -344,8 +351,15 @@ def make_protein(n_atoms: int, seed: int, radius: float | None = None) -> Synthe
         cosang = max(-1.0, min(1.0, cosang))
         alpha = math.acos(cosang)
         v, w = _perp_basis(u, rng)
-        ph = rng.uniform(0, 2 * math.pi)
-        dirc = math.cos(alpha) * u + math.sin(alpha) * (math.cos(ph) * v + math.sin(ph) * w)
+        ph = rng.uniform(0, 2 * math.pi) + np.linspace(0.0, 2 * math.pi, 36, endpoint=False)
+        dirs_c = (math.cos(alpha) * u[None, :] + math.sin(alpha) * (np.cos(ph)[:, None] * v[None, :]
+                                                                      + np.sin(ph)[:, None] * w[None, :]))
+        if placed[bb[r]["N"]]:
+            # choose the phase that puts the N-CA-C angle nearest 111 degrees
+            nv = _unit(pos[bb[r]["N"]] - a)
+            dirc = dirs_c[int(np.argmin(np.abs(dirs_c @ nv - math.cos(math.radians(111.0)))))]
+        else:
+            dirc = dirs_c[0]
         cpos = a + b_cac * dirc
         put(bb[r]["C"], cpos)
         ax = b - cpos
@@ -354,8 +368,13 @@ def make_protein(n_atoms: int, seed: int, radius: float | None = None) -> Synthe
         x = (dd**2 + b_cn**2 - b_nca**2) / (2 * dd)
         h = math.sqrt(max(b_cn**2 - x**2, 0.0))
         v2, w2 = _perp_basis(ax, rng)
-        ph2 = rng.uniform(0, 2 * math.pi)
-        npos = cpos + x * ax + h * (math.cos(ph2) * v2 + math.sin(ph2) * w2)
+        ph2 = rng.uniform(0, 2 * math.pi) + np.linspace(0.0, 2 * math.pi, 36, endpoint=False)
+        n_cands = cpos[None, :] + x * ax[None, :] + h * (np.cos(ph2)[:, None] * v2[None, :]
+                                                       + np.sin(ph2)[:, None] * w2[None, :])
+        # the phase that puts the CA-C-N angle nearest 116 degrees
+        uc = _unit(a - cpos)
+        un = (n_cands - cpos[None, :]) / np.linalg.norm(n_cands - cpos[None, :], axis=1, keepdims=True)
+        npos = n_cands[int(np.argmin(np.abs(un @ uc - math.cos(math.radians(116.0)))))]
         put(bb[r + 1]["N"], npos)
     # Chain termini.
     first_n = bb[0]["N"]
@@ -439,7 +458,7 @@ def make_protein(n_atoms: int, seed: int, radius: float | None = None) -> Synthe
         v1 = pos[angles[:, 0]] - pos[angles[:, 1]]
         v2 = pos[angles[:, 2]] - pos[angles[:, 1]]
         ct = np.sum(v1 * v2, 1) / (np.linalg.norm(v1, axis=1) * np.linalg.norm(v2, axis=1))
-        th0 = np.arccos(np.clip(ct, -1.0, 1.0))
+        th0 = np.clip(np.arccos(np.clip(ct, -1.0, 1.0)), math.radians(ANGLE_MIN_DEG), math.radians(ANGLE_MAX_DEG))
     else:
         th0 = np.zeros(0)
     angle_params = np.stack([th0, rng.uniform(80.0, 140.0, size=len(angles))], axis=1)

@@ -3,8 +3,8 @@
 Tolerances: energy relative error <= 1e-6 and force RMS relative error
 <= 1e-5 against the float64 oracle on the same float32-rounded inputs. The
 energy error is taken relative to max(|E_total|, 0.2 * sum_k |E_k|), which is
-|E_total| for every benchmark configuration (peptide: 873 of 2043; globule:
-22054 of 32311) and only differs for small systems whose total is a near
+|E_total| for every benchmark configuration (peptide: 1574 of 2731; globule:
+27561 of 36302) and only differs for small systems whose total is a near
 cancellation of its terms.
 """
 

@@ -262,3 +262,30 @@ def test_gb_cutoff_separated_ions_are_isolated():
     assert O.energy_terms(s, x, cutoff=5.0)["gb"] != pytest.approx(t["gb"], rel=1e-6)
     f = O.forces(s, x, cutoff=5.0, gb_cutoff=True)
     assert np.max(np.abs(f)) == 0.0
+
+
+# ----------------------------------------------------------------------- FIRE
+
+def test_fire_converges_in_a_harmonic_bowl():
+    """FIRE on E = 1/2 sum k_i |x_i - c_i|^2 reaches the minimum; dt stays <= dt_max."""
+    rng = np.random.default_rng(0)
+    c = rng.normal(size=(7, 3))
+    k = rng.uniform(5.0, 50.0, size=(7, 1))
+    force = lambda x: -k * (x - c)
+    x0 = c + rng.normal(size=c.shape)
+    x, p, f, dt, alpha, n_pos = O.fire_minimize(force, x0, np.full(7, 12.0), 2000, 0.01, 0.1)
+    assert np.max(np.abs(x - c)) < 1e-6
+    assert dt <= 0.1 + 1e-15
+
+
+def test_fire_caps_bound_every_step():
+    """With a force cap, no atom moves more than disp_cap in one step, even from a
+    steep start (simulate.py:615-640)."""
+    rng = np.random.default_rng(1)
+    c = rng.normal(size=(5, 3))
+    force = lambda x: -1e5 * (x - c)
+    x = c + rng.normal(size=c.shape)
+    for _ in range(20):
+        x_new = O.fire_minimize(force, x, np.ones(5), 1, 0.05, 0.5, force_cap=10.0, disp_cap=0.1)[0]
+        assert np.max(np.linalg.norm(x_new - x, axis=1)) <= 0.1 + 1e-12
+        x = x_new
