@@ -40,7 +40,7 @@ constexpr int kClsStraddle = 1;  // per-pair r^2 < cutoff^2 test needed
 constexpr int kClsFar = 2;       // every pair at or beyond the cutoff
 constexpr int kClsCutMask = 3;
 // Bit 2: every pair of the item is in the OBC2 far field in both directions
-// (r - s_j > rho_i, s_j <= 0.6 r and the same with i, j swapped), so the Born
+// (r - s_j > rho_i, s_j <= r/2 and the same with i, j swapped), so the Born
 // and chain passes use the far-field polynomials without a per-pair test.
 constexpr int kClsObcFar = 4;
 

@@ -147,7 +147,7 @@ __device__ __forceinline__ float born_dr(float r, float rs, float rho, float s, 
   return on * (d1 + d2 + d3 + d4);
 }
 
-// Far field (r - s > rho and s <= 0.6 r): L = r - s, U = r + s and
+// Far field (r - s > rho and s <= r/2): L = r - s, U = r + s and
 //   I = x^3 h(x^2) / (2r),  dI/dr = -x^3 k(x^2) / r^2,  x = s / r,
 // with the polynomials of prx_obc_farfield.h. Same function as the direct
 // formula, without its cancellation and without a transcendental.
@@ -167,12 +167,24 @@ __device__ __forceinline__ float obc_far_I(float s2, float s3h, float u, float u
   return (s3h * u2) * h;
 }
 
-__device__ __forceinline__ float obc_far_dI(float s2, float s3h, float u, float u2r) {
-  const float y = s2 * u;
-  float q = kObcK[kObcKDeg];
+// -2 k(y): the factor -2 scales every coefficient exactly, so this is -2 times
+// the Horner value of k bit for bit.
+__device__ __forceinline__ float obc_far_km2(float y) {
+  float q = -2.0f * kObcK[kObcKDeg];
 #pragma unroll
-  for (int k = kObcKDeg - 1; k >= 0; --k) q = fmaf(q, y, kObcK[k]);
-  return (-2.0f * s3h * u2r) * q;
+  for (int k = kObcKDeg - 1; k >= 0; --k) q = fmaf(q, y, -2.0f * kObcK[k]);
+  return q;
+}
+
+__device__ __forceinline__ float obc_far_dI(float s2, float s3h, float u, float u2r) {
+  return (s3h * u2r) * obc_far_km2(s2 * u);
+}
+
+// Chain-rule scalar of a pair with both directions in the far field:
+//   (c_i dI(rho_i, s_j)/dr + c_j dI(rho_j, s_i)/dr) / r
+//     = -2 u^3 (c_i s_j^3/2 k(s_j^2 u) + c_j s_i^3/2 k(s_i^2 u)),   u = 1/r^2.
+__device__ __forceinline__ float chain_far(float u, float sj2, float ci_s3hj, float si2, float cj_s3hi) {
+  return (u * u * u) * fmaf(ci_s3hj, obc_far_km2(sj2 * u), cj_s3hi * obc_far_km2(si2 * u));
 }
 
 // ---------------------------------------------------------------------------
@@ -217,13 +229,14 @@ __device__ __forceinline__ void born_tile(const DeviceView& v, const PairItem& i
     const float xj = ja.x, yj = ja.y, zj = ja.z, sj2 = ja.w, sj3h = L.b[lane + k].x;
     const float dx = xj - pi.x, dy = yj - pi.y, dz = zj - pi.z;
     const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-    const float rinv = frsq(r2);
-    const float u = rinv * rinv, u2 = u * u;
     BornPair bp;
     if (FAR) {
+      const float u = frcp(r2), u2 = u * u;
       bp.I1 = obc_far_I(sj2, sj3h, u, u2);
       bp.I2 = obc_far_I(bi.z, bi.w, u, u2);
     } else {
+      const float rinv = frsq(r2);
+      const float u = rinv * rinv, u2 = u * u;
       const float rhoj = L.b[lane + k].y, sj = L.b[lane + k].z;
       const float r = r2 * rinv;
       const bool far1 = obc_far(r, bi.x, sj), far2 = obc_far(r, rhoj, bi.y);
@@ -265,7 +278,7 @@ __device__ __forceinline__ void born_tile(const DeviceView& v, const PairItem& i
 
 // Class bits of an item from its tiles' bounding boxes (wave-uniform). The
 // OBC2 far-field bit needs the box distance above every pair's threshold,
-// max(s_j + rho_i, s_i + rho_j, s / 0.6) over the two tiles, with a 1e-4
+// max(s_j + rho_i, s_i + rho_j, 2 s) over the two tiles, with a 1e-4
 // relative margin on the squared distance.
 __device__ __forceinline__ int classify(const DeviceView& v, const SlotConst& S, const PairItem& it) {
   const int tI = S.tile0 + ((it.i0 - S.a0) >> 6);
@@ -545,24 +558,24 @@ __device__ __forceinline__ void chain_tile(const DeviceView& v, const PairItem& 
     const float xj = ja.x, yj = ja.y, zj = ja.z, sj2 = ja.w, sj3h = jb.x, cj = jb.y;
     const float dx = xj - pi.x, dy = yj - pi.y, dz = zj - pi.z;
     const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-    const float rinv = frsq(r2);
-    const float rhoj = L.b[lane + k].z, sj = L.b[lane + k].w;  // near field only
-    const float r = r2 * rinv;
-    const bool far1 = obc_far(r, bi.x, sj), far2 = obc_far(r, rhoj, bi.y);
-    float d1, d2;  // dI(r; rho_i, s_j)/dr and dI(r; rho_j, s_i)/dr
-    const float u = rinv * rinv, u2r = u * u * rinv;
+    float gsc;  // (c_i dI(r; rho_i, s_j)/dr + c_j dI(r; rho_j, s_i)/dr) / r; F_i = gsc (x_j - x_i)
     if (FAR) {
-      d1 = obc_far_dI(sj2, sj3h, u, u2r);
-      d2 = obc_far_dI(bi.z, bi.w, u, u2r);
-    } else if (__all(far1 && far2)) {
-      d1 = obc_far_dI(sj2, sj3h, u, u2r);
-      d2 = obc_far_dI(bi.z, bi.w, u, u2r);
+      gsc = chain_far(frcp(r2), sj2, ci * sj3h, bi.z, cj * bi.w);
     } else {
-      const float rs = fminf(rinv, 1e4f);
-      d1 = far1 ? obc_far_dI(sj2, sj3h, u, u2r) : born_dr(r, rs, bi.x, sj, sj2);
-      d2 = far2 ? obc_far_dI(bi.z, bi.w, u, u2r) : born_dr(r, rs, rhoj, bi.y, bi.z);
+      const float rinv = frsq(r2);
+      const float rhoj = L.b[lane + k].z, sj = L.b[lane + k].w;  // near field only
+      const float r = r2 * rinv;
+      const bool far1 = obc_far(r, bi.x, sj), far2 = obc_far(r, rhoj, bi.y);
+      const float u = rinv * rinv;
+      if (__all(far1 && far2)) {
+        gsc = chain_far(u, sj2, ci * sj3h, bi.z, cj * bi.w);
+      } else {
+        const float u2r = u * u * rinv, rs = fminf(rinv, 1e4f);
+        const float d1 = far1 ? obc_far_dI(sj2, sj3h, u, u2r) : born_dr(r, rs, bi.x, sj, sj2);
+        const float d2 = far2 ? obc_far_dI(bi.z, bi.w, u, u2r) : born_dr(r, rs, rhoj, bi.y, bi.z);
+        gsc = fmaf(ci, d1, cj * d2) * rinv;
+      }
     }
-    float gsc = fmaf(ci, d1, cj * d2) * rinv;           // F_i = gsc * (x_j - x_i)
     if (GENERAL) {
       const int orig = (lane + k) & 63;
       bool valid = ivalid && orig < nj;

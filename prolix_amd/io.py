@@ -1,0 +1,193 @@
+"""Simulation-state records and trajectory files.
+
+Mirrors prolix's ``SimulationState`` (prolix/typing.py:653-728) and
+``TrajectoryWriter`` (prolix/simulate.py:148-203):
+
+- a state is positions, velocities, step and time (ns), optionally forces,
+  masses and the potential / kinetic energies;
+- ``to_array_record`` packs ``numpy()`` minus its None fields with msgpack.
+  Arrays and numpy scalars use the wire format of msgpack-numpy 0.4.8 (the
+  reference's pinned encoder, not installed here), restated in
+  ``_encode`` / ``_decode``: an array becomes the map
+  {b"nd": True, b"type": dtype.str, b"kind": b"", b"shape": shape, b"data": bytes},
+  and a numpy scalar becomes {b"nd": False, b"type": dtype.str, b"data": bytes}.
+  Records written here therefore unpack with msgpack-numpy, and theirs with
+  ``from_array_record``;
+- ArrayRecord (the reference's container, ``array_record``) is not
+  installed either. ``TrajectoryWriter`` stores the same per-frame records
+  in a length-prefixed file: the magic ``PRXTRJ1\\n``, then per record a
+  little-endian uint64 length and the bytes.
+
+Velocities of engine states are in A per AKMA time unit (48.88821291839 fs),
+the units the engine's integrator runs in.
+"""
+
+from __future__ import annotations
+
+import struct
+from dataclasses import dataclass
+from typing import Any, Iterator, Sequence
+
+import msgpack
+import numpy as np
+
+MAGIC = b"PRXTRJ1\n"
+
+
+def _encode(obj):
+    if isinstance(obj, np.ndarray):
+        if obj.dtype.kind == "V":
+            raise TypeError("structured arrays are not part of simulation states")
+        return {b"nd": True, b"type": obj.dtype.str, b"kind": b"", b"shape": list(obj.shape),
+                b"data": np.ascontiguousarray(obj).tobytes()}
+    if isinstance(obj, (np.bool_, np.number)):
+        return {b"nd": False, b"type": obj.dtype.str, b"data": obj.tobytes()}
+    return obj
+
+
+def _decode(obj):
+    if b"nd" in obj:
+        if obj[b"nd"] is True:
+            if obj.get(b"kind") == b"V":
+                raise TypeError("structured arrays are not part of simulation states")
+            return np.ndarray(buffer=obj[b"data"], dtype=np.dtype(obj[b"type"]), shape=tuple(obj[b"shape"])).copy()
+        return np.frombuffer(obj[b"data"], dtype=np.dtype(obj[b"type"]))[0]
+    if b"complex" in obj:
+        data = obj[b"data"]
+        return complex(data.decode() if isinstance(data, bytes) else data)
+    return obj
+
+
+def packb(data: Any) -> bytes:
+    return msgpack.packb(data, default=_encode, use_bin_type=True)
+
+
+def unpackb(packed: bytes) -> Any:
+    return msgpack.unpackb(packed, object_hook=_decode, raw=False)
+
+
+@dataclass
+class SimulationState:
+    """One frame (prolix/typing.py:653-728)."""
+
+    positions: Any
+    velocities: Any
+    step: Any
+    time_ns: Any
+    forces: Any = None
+    mass: Any = None
+    potential_energy: Any = None
+    kinetic_energy: Any = None
+
+    @property
+    def position(self):
+        return self.positions
+
+    def numpy(self) -> dict:
+        def cpu(x):
+            if x is None or isinstance(x, (int, float)):
+                return x
+            if hasattr(x, "detach"):
+                x = x.detach().cpu().numpy()
+            return np.asarray(x)
+
+        return {k: cpu(getattr(self, k)) for k in ("positions", "velocities", "forces", "mass", "step", "time_ns",
+                                                     "potential_energy", "kinetic_energy")}
+
+    def to_array_record(self) -> bytes:
+        return packb({k: v for k, v in self.numpy().items() if v is not None})
+
+    @classmethod
+    def from_array_record(cls, packed: bytes) -> "SimulationState":
+        d = unpackb(packed)
+        return cls(positions=d["positions"], velocities=d["velocities"], step=d["step"], time_ns=d["time_ns"],
+                   forces=d.get("forces"), mass=d.get("mass"), potential_energy=d.get("potential_energy"),
+                   kinetic_energy=d.get("kinetic_energy"))
+
+
+class TrajectoryWriter:
+    """Appends per-frame state records to a trajectory file (prolix/simulate.py:148-203)."""
+
+    def __init__(self, path: str):
+        self._fh = open(path, "wb")
+        self._fh.write(MAGIC)
+        self.closed = False
+
+    def _put(self, rec: bytes):
+        if self.closed:
+            raise RuntimeError("Writer is closed")
+        self._fh.write(struct.pack("<Q", len(rec)))
+        self._fh.write(rec)
+
+    def write(self, states: SimulationState | Sequence[SimulationState]) -> None:
+        if isinstance(states, SimulationState):
+            states = [states]
+        for st in states:
+            self._put(st.to_array_record())
+
+    def write_batch(self, stacked: SimulationState) -> None:
+        """One record per leading index of a stacked state (scalars are shared)."""
+        data = stacked.numpy()
+        n = np.asarray(data["positions"]).shape[0]
+        for i in range(n):
+            frame = {k: (v[i] if v is not None and getattr(v, "ndim", 0) > 0 else v) for k, v in data.items()
+                     if v is not None}
+            self._put(packb(frame))
+
+    def close(self) -> None:
+        if not self.closed:
+            self._fh.close()
+            self.closed = True
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        if not getattr(self, "closed", True):
+            self.close()
+
+
+def read_trajectory(path: str) -> Iterator[SimulationState]:
+    """Frames of a file written by TrajectoryWriter, in order."""
+    with open(path, "rb") as fh:
+        if fh.read(len(MAGIC)) != MAGIC:
+            raise ValueError(f"{path}: not a trajectory file")
+        while True:
+            head = fh.read(8)
+            if not head:
+                return
+            if len(head) != 8:
+                raise ValueError(f"{path}: truncated record header")
+            (n,) = struct.unpack("<Q", head)
+            rec = fh.read(n)
+            if len(rec) != n:
+                raise ValueError(f"{path}: truncated record")
+            yield SimulationState.from_array_record(rec)
+
+
+def engine_states(engine, masses_amu, dt_fs: float, energies: bool = True) -> list[SimulationState]:
+    """One state per system of a GBEngine, read from its device-resident integrator state.
+
+    ``masses_amu`` are the packed per-atom masses the integrator runs with;
+    velocities are momentum / mass.
+    """
+    m = np.asarray(masses_amu, np.float64).reshape(-1)
+    if m.shape[0] != engine.n_atoms:
+        raise ValueError("one mass per packed atom is required")
+    x, p, f = engine.state()
+    xs, ps, fs = engine.unpack(x), engine.unpack(p), engine.unpack(f)
+    e = engine.energy_forces(x)[0].cpu().numpy() if energies else None
+    ke = engine.kinetic().cpu().numpy() if energies else None
+    out = []
+    for k in range(engine.n_systems):
+        mk = m[engine.offsets[k]:engine.offsets[k + 1]]
+        steps = engine.step_count(k)
+        out.append(SimulationState(
+            positions=np.asarray(xs[k], np.float32), velocities=(ps[k] / mk[:, None]).astype(np.float32),
+            step=np.int64(steps), time_ns=np.float64(steps * dt_fs * 1e-6), forces=np.asarray(fs[k], np.float32),
+            mass=mk, potential_energy=None if e is None else np.float64(e[k, -1]),
+            kinetic_energy=None if ke is None else np.float64(ke[k])))
+    return out

@@ -94,7 +94,10 @@ def test_far_field_polynomials_match_series():
     txt = (ROOT / "prolix_amd/csrc/prx_obc_farfield.h").read_text()
     H = [float(v.rstrip("f")) for v in re.search(r"kObcH\[\d+\] = \{(.*?)\}", txt).group(1).split(",")]
     K = [float(v.rstrip("f")) for v in re.search(r"kObcK\[\d+\] = \{(.*?)\}", txt).group(1).split(",")]
-    y = np.linspace(0.0, 0.36, 2001)
+    ymax = float(re.search(r"PRX_OBC_FAR_YMAX ([0-9.]+)f", txt).group(1))
+    xmax = float(re.search(r"PRX_OBC_FAR_XMAX ([0-9.]+)f", txt).group(1))
+    assert xmax * xmax == pytest.approx(ymax, rel=1e-7)
+    y = np.linspace(0.0, ymax, 2001)
     h = sum(2 * k / (2 * k + 1) * y ** (k - 1) for k in range(1, 300))
     kk = sum(2 * k * (k + 1) / (2 * k + 1) * y ** (k - 1) for k in range(1, 300))
     assert np.max(np.abs(np.polyval(H[::-1], y) / h - 1)) < 2e-7
@@ -103,7 +106,7 @@ def test_far_field_polynomials_match_series():
     rng = np.random.default_rng(1)
     for _ in range(100):
         s, rho = rng.uniform(0.8, 1.8), rng.uniform(1.1, 1.9)
-        r = max(s + rho, s / 0.6) + rng.uniform(0.01, 30)
+        r = max(s + rho, s / xmax) + rng.uniform(0.01, 30)
         x = s / r
         poly_I = 0.5 * x**3 / r * np.polyval(H[::-1], x * x)
         poly_dI = -(x**3) / r**2 * np.polyval(K[::-1], x * x)
