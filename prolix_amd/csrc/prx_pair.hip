@@ -62,11 +62,19 @@ __device__ __forceinline__ void wave_lds_sync() {
 
 constexpr int kWavesPerBlock = 4;
 
+// LDS j accumulators are read and written through volatile pointers: the
+// compiler keeps those accesses in program order (the next iteration's read of
+// an entry another lane just wrote stays behind that write) while the
+// read-only j data loads remain free to move.
+typedef float v4f __attribute__((ext_vector_type(4)));
+
 // Build-time tuning knobs (defaults are the measured best on MI355X):
 // unroll depth of the three pair loops, and where the j-side accumulators
-// live: 0 = registers rotated by DPP; 1 = LDS, entry (l + k) mod 64, added
-// with ds_add_f32 in iteration order (deterministic: one add per entry per
-// iteration, a wave's LDS operations complete in issue order).
+// live: 0 = registers rotated by DPP; 1 = LDS, stored twice over like the j
+// data: lane l adds iteration k's term to entry l + k (an immediate offset;
+// no two lanes share an entry in one iteration, and a wave's LDS operations
+// complete in issue order, so the read-add-write needs only a compiler
+// ordering), and atom j's sum is entry j plus entry j + 64.
 #ifndef PRX_UNROLL_BORN
 #define PRX_UNROLL_BORN 8
 #endif
@@ -196,7 +204,7 @@ struct BornLds {  // (x, y, z, s^2), (s^3/2, rho, s, 0); j accumulators
   float4 a[128];
   float4 b[128];
 #if PRX_LDS_ACC
-  float acc[64];
+  float acc[128];
 #endif
 };
 
@@ -217,6 +225,7 @@ __device__ __forceinline__ void born_tile(const DeviceView& v, const PairItem& i
     L.b[lane] = b; L.b[lane + 64] = b;
 #if PRX_LDS_ACC
     L.acc[lane] = 0.0f;
+    L.acc[lane + 64] = 0.0f;
 #endif
   }
   wave_lds_sync();
@@ -259,7 +268,10 @@ __device__ __forceinline__ void born_tile(const DeviceView& v, const PairItem& i
     }
     accI += bp.I1;
 #if PRX_LDS_ACC
-    atomicAdd(&L.acc[(lane + k) & 63], bp.I2);
+    {
+      volatile float* acc = L.acc;
+      acc[lane + k] = acc[lane + k] + bp.I2;
+    }
 #else
     accJ += bp.I2;
     if (k != kend) accJ = rot(accJ);
@@ -269,7 +281,7 @@ __device__ __forceinline__ void born_tile(const DeviceView& v, const PairItem& i
   pb[part_slot(w, 0, lane)] = accI;
 #if PRX_LDS_ACC
   wave_lds_sync();
-  pb[part_slot(w, 1, lane)] = L.acc[lane];
+  pb[part_slot(w, 1, lane)] = L.acc[lane] + L.acc[lane + 64];
 #else
   const int home = (lane + kend) & 63;
   pb[part_slot(w, 1, home)] = accJ;
@@ -344,7 +356,7 @@ struct PairLds {  // two float4 of j data per entry; j accumulators
   float4 a[128];
   float4 b[128];
 #if PRX_LDS_ACC
-  float4 acc[64];
+  float4 acc[128];
 #endif
 };
 
@@ -369,6 +381,7 @@ __device__ __forceinline__ void force_tile(const DeviceView& v, const PairItem& 
     L.b[lane] = b; L.b[lane + 64] = b;
 #if PRX_LDS_ACC
     L.acc[lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+    L.acc[lane + 64] = make_float4(0.f, 0.f, 0.f, 0.f);
 #endif
   }
   wave_lds_sync();
@@ -450,11 +463,10 @@ __device__ __forceinline__ void force_tile(const DeviceView& v, const PairItem& 
     dbi = fmaf(g, Bj, dbi);
 #if PRX_LDS_ACC
     {
-      float* ap = reinterpret_cast<float*>(&L.acc[(lane + k) & 63]);
-      atomicAdd(ap + 0, fr * dx);
-      atomicAdd(ap + 1, fr * dy);
-      atomicAdd(ap + 2, fr * dz);
-      atomicAdd(ap + 3, -0.5f * g * gbi.x);
+      volatile v4f* acc = reinterpret_cast<volatile v4f*>(L.acc);
+      v4f t = acc[lane + k];
+      t.x = fmaf(fr, dx, t.x); t.y = fmaf(fr, dy, t.y); t.z = fmaf(fr, dz, t.z); t.w = fmaf(g, gbi.x, t.w);
+      acc[lane + k] = t;
     }
 #else
     fxj = fmaf(fr, dx, fxj); fyj = fmaf(fr, dy, fyj); fzj = fmaf(fr, dz, fzj);
@@ -467,7 +479,10 @@ __device__ __forceinline__ void force_tile(const DeviceView& v, const PairItem& 
   v.part[part_slot(w, 0, lane)] = make_float4(fxi, fyi, fzi, -0.5f * dbi);
 #if PRX_LDS_ACC
   wave_lds_sync();
-  v.part[part_slot(w, 1, lane)] = L.acc[lane];
+  {
+    const float4 a0 = L.acc[lane], a1 = L.acc[lane + 64];
+    v.part[part_slot(w, 1, lane)] = make_float4(a0.x + a1.x, a0.y + a1.y, a0.z + a1.z, -0.5f * (a0.w + a1.w));
+  }
 #else
   const int home = (lane + kend) & 63;
   v.part[part_slot(w, 1, home)] = make_float4(fxj, fyj, fzj, -0.5f * dbj);
@@ -545,6 +560,7 @@ __device__ __forceinline__ void chain_tile(const DeviceView& v, const PairItem& 
     L.b[lane] = b; L.b[lane + 64] = b;
 #if PRX_LDS_ACC
     L.acc[lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+    L.acc[lane + 64] = make_float4(0.f, 0.f, 0.f, 0.f);
 #endif
   }
   wave_lds_sync();
@@ -586,10 +602,10 @@ __device__ __forceinline__ void chain_tile(const DeviceView& v, const PairItem& 
     fxi = fmaf(gsc, dx, fxi); fyi = fmaf(gsc, dy, fyi); fzi = fmaf(gsc, dz, fzi);
 #if PRX_LDS_ACC
     {
-      float* ap = reinterpret_cast<float*>(&L.acc[(lane + k) & 63]);
-      atomicAdd(ap + 0, -gsc * dx);
-      atomicAdd(ap + 1, -gsc * dy);
-      atomicAdd(ap + 2, -gsc * dz);
+      volatile v4f* acc = reinterpret_cast<volatile v4f*>(L.acc);
+      v4f t = acc[lane + k];
+      t.x = fmaf(-gsc, dx, t.x); t.y = fmaf(-gsc, dy, t.y); t.z = fmaf(-gsc, dz, t.z);
+      acc[lane + k] = t;
     }
 #else
     fxj = fmaf(-gsc, dx, fxj); fyj = fmaf(-gsc, dy, fyj); fzj = fmaf(-gsc, dz, fzj);
@@ -601,7 +617,10 @@ __device__ __forceinline__ void chain_tile(const DeviceView& v, const PairItem& 
   v.part[part_slot(w, 0, lane)] = make_float4(fxi, fyi, fzi, 0.0f);
 #if PRX_LDS_ACC
   wave_lds_sync();
-  v.part[part_slot(w, 1, lane)] = L.acc[lane];
+  {
+    const float4 a0 = L.acc[lane], a1 = L.acc[lane + 64];
+    v.part[part_slot(w, 1, lane)] = make_float4(a0.x + a1.x, a0.y + a1.y, a0.z + a1.z, 0.0f);
+  }
 #else
   const int home = (lane + kend) & 63;
   v.part[part_slot(w, 1, home)] = make_float4(fxj, fyj, fzj, 0.0f);
