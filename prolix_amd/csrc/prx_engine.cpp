@@ -35,6 +35,11 @@ struct StateError : std::runtime_error {
   using std::runtime_error::runtime_error;
 };
 
+// Bonded terms on a side stream beside the pair passes (1) or in line (0).
+#ifndef PRX_FORK_BONDED
+#define PRX_FORK_BONDED 1
+#endif
+
 #define HIPCHECK(x)                                                                          \
   do {                                                                                       \
     hipError_t e_ = (x);                                                                     \
@@ -233,6 +238,8 @@ struct prx_engine {
   std::vector<int32_t> order;   // live slots in packed order
   std::vector<int32_t> offsets; // packed atom offsets of `order`
   hipStream_t stream = nullptr;
+  hipStream_t side = nullptr;  // bonded terms, forked from and joined to the pass stream
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   bool dirty = true;            // work list / ranges need rebuilding
   bool initialized = false;
   std::vector<std::pair<int, hipGraphExec_t>> graphs;  // (steps per launch, executable)
@@ -286,6 +293,9 @@ struct prx_engine {
     cap_map = cap_atoms;
     d_map = dalloc<int32_t>(cap_map);
     HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    HIPCHECK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+    HIPCHECK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+    HIPCHECK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
   }
 
   template <class E>
@@ -313,6 +323,9 @@ struct prx_engine {
                     d_tbox, d_item_cls};
     for (void* p : ptrs)
       if (p) hipFree(p);
+    if (ev_fork) hipEventDestroy(ev_fork);
+    if (ev_join) hipEventDestroy(ev_join);
+    if (side) hipStreamDestroy(side);
     if (stream) hipStreamDestroy(stream);
   }
 
@@ -645,14 +658,30 @@ struct prx_engine {
 
   // ---- evaluation -------------------------------------------------------
   // Everything of one force evaluation except the final chain gather.
+  // Everything up to the chain gather. The bonded terms read only positions
+  // and write their records and term energies, which only the chain gather
+  // and the energy reduction read: with PRX_FORK_BONDED they run on the side
+  // stream beside the pair passes and join before this returns (in a
+  // captured graph, a parallel branch).
   void force_passes(bool energy, hipStream_t s) {
+#if PRX_FORK_BONDED
+    HIPCHECK(hipEventRecord(ev_fork, s));
+    HIPCHECK(hipStreamWaitEvent(side, ev_fork, 0));
+    prx::launch_bonded(v, energy, side);
+    HIPCHECK(hipEventRecord(ev_join, side));
+#endif
     prx::launch_tile_boxes(v, s);
     prx::launch_born_pairs(v, s);
     prx::launch_born_gather(v, s);
     prx::launch_force_pairs(v, energy, s);
     prx::launch_force_gather(v, energy, s);
+#if PRX_FORK_BONDED
+    prx::launch_chain_pairs(v, s);
+    HIPCHECK(hipStreamWaitEvent(s, ev_join, 0));
+#else
     prx::launch_bonded(v, energy, s);
     prx::launch_chain_pairs(v, s);
+#endif
   }
 
   void evaluate(bool energy, hipStream_t s) {
