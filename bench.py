@@ -38,7 +38,9 @@ sys.path.insert(0, ROOT)
 METRIC = "aggregate ns/day + pair-interactions/s, batched GB-OBC2 MD; p50/p99 latency"
 FLOPS_PER_PAIR = {"born_pairs": 56, "force_pairs": 73, "chain_pairs": 62}  # algorithmic, per unordered pair
 FLOPS_TOTAL = 191
-PEAK_FP32 = 157.3  # TFLOP/s, MI355X FP32 vector peak
+PEAK_FP32 = 157.3  # TFLOP/s, MI355X FP32 vector peak (packed FP32)
+PEAK_FP32_UNPACKED = 78.6  # TFLOP/s, one FP32 FMA per lane per clock (the kernels issue above it: plain
+#                            v_fma_f32 measured at 1.36 wave64 instructions per clock per CU)
 DT_FS = 1.0
 
 
@@ -50,9 +52,10 @@ def ns_per_day(steps_per_s):
     return steps_per_s * DT_FS * 1e-6 * 86400.0
 
 
-def cpu_baseline(systems, sizes_full, budget_s=15.0, n_sample=16):
+def cpu_baseline(systems, sizes_full, budget_s=20.0, n_sample=16, repeats=5):
     """Time the C++ restatement (float32, all host cores) on a seeded subset of
-    the same mix and scale by sum N^2 to the whole mix."""
+    the same mix and scale by sum N^2 to the whole mix: one warm-up step, then
+    the median of ``repeats`` timed runs sized to share ``budget_s``."""
     from oracle import cpu as C
 
     rng = np.random.default_rng(11)
@@ -72,8 +75,8 @@ def cpu_baseline(systems, sizes_full, budget_s=15.0, n_sample=16):
         return time.perf_counter() - t
 
     t1 = run(1)
-    steps = max(1, min(50, int(budget_s / max(t1, 1e-3))))
-    t = run(steps)
+    steps = max(1, min(50, int(budget_s / repeats / max(t1, 1e-3))))
+    t = float(np.median([run(steps) for _ in range(repeats)]))
     sub_sizes = np.array([len(s["charges"]) for s in sub], np.float64)
     sub_pairs = float(np.sum(sub_sizes * (sub_sizes - 1) / 2))
     full_pairs = float(np.sum(sizes_full * (sizes_full - 1.0) / 2))
@@ -83,10 +86,55 @@ def cpu_baseline(systems, sizes_full, budget_s=15.0, n_sample=16):
         "unit": "ns/day",
         "cores": C.threads(),
         "kind": "port",
-        "sample": (f"{len(sub)} of the {len(sizes_full)} systems ({int(sub_sizes.sum())} atoms), {steps} BAOAB steps "
-                   f"each in float32 ({t:.1f} s); scaled to the full mix by sum N(N-1)/2"),
+        "sample": (f"{len(sub)} of the {len(sizes_full)} systems ({int(sub_sizes.sum())} atoms), median of "
+                   f"{repeats} runs of {steps} BAOAB steps each in float32 ({t:.2f} s per run, after one warm-up "
+                   "step); scaled to the full mix by sum N(N-1)/2"),
         "pair_interactions_per_s": sub_pairs * steps / t,
     }
+
+
+def single_system(sysd, gpu_steps, cpu_steps, label, with_cpu, repeats=3):
+    """One system alone: the plain GPU path (its own engine, graph-captured
+    steps) and, beside it, the C++ restatement on the host cores."""
+    import torch
+
+    from prolix_amd.engine import GBEngine, Langevin
+    from prolix_amd.minimize import minimize
+
+    n = len(sysd["charges"])
+    eng = GBEngine(sysd, cutoff=0.0)
+    x = minimize(eng, sysd["positions"], steps=100)
+    eng.init(x, Langevin(DT_FS, 300.0, 1.0, seed=3, safety_caps=True))
+    eng.steps(100)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    eng.steps(gpu_steps)
+    torch.cuda.synchronize()
+    g = (time.perf_counter() - t) / gpu_steps
+    eng.close()
+    out = {"workload": label, "atoms": n, "gpu_steps": gpu_steps, "gpu_ms_per_step": g * 1e3,
+           "gpu_ns_per_day": ns_per_day(1.0 / g), "gpu_pair_interactions_per_s": n * (n - 1) / 2 / g}
+    if with_cpu:
+        from oracle import cpu as C
+
+        c = C.CpuSystem(sysd)
+        xc = np.ascontiguousarray(x.cpu().numpy(), np.float32)
+        _, f = c.energy_forces(xc, np.float32)
+        pc = np.zeros_like(xc)
+        c.baoab(xc, pc, f, 1, DT_FS, 300.0, 1.0, 7, dtype=np.float32)  # warm-up
+
+        def run():
+            t0 = time.perf_counter()
+            c.baoab(xc, pc, f, cpu_steps, DT_FS, 300.0, 1.0, 7, dtype=np.float32)
+            return time.perf_counter() - t0
+
+        tc = float(np.median([run() for _ in range(repeats)])) / cpu_steps
+        out.update({"cpu_steps": cpu_steps, "cpu_ms_per_step": tc * 1e3, "cpu_ns_per_day": ns_per_day(1.0 / tc),
+                    "cpu_cores": C.threads(), "cpu_kind": "port",
+                    "cpu_sample": f"median of {repeats} runs of {cpu_steps} float32 BAOAB steps after one warm-up"
+                                  + ("" if cpu_steps >= gpu_steps else f"; per-step time extrapolated to {gpu_steps} steps"),
+                    "gpu_over_cpu": tc / g})
+    return out
 
 
 def main():
@@ -101,6 +149,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-large", action="store_true")
+    ap.add_argument("--no-single", action="store_true",
+                    help="skip the one-system legs (138-atom peptide, 2,500-atom globule)")
     ap.add_argument("--calibration-copy-mb", type=float, default=0.0,
                     help="also run one elementwise copy of this many MiB (float32), the reference kernel "
                          "against which rocprofv3 FETCH_SIZE / WRITE_SIZE readings are scaled")
@@ -125,6 +175,12 @@ def main():
     prots, temps = built[:n_mix], synth.mix_temperatures(a.n_systems, seed=2)
     serve_prots = built[n_mix:n_mix + (0 if a.no_serve else a.serve_requests)]
     large = None if a.no_large else built[-1]
+    singles = []
+    if not a.no_single:
+        singles = [("peptide: 138 atoms (10 residues), no cutoff, BAOAB 300 K, gamma 1/ps, dt 1 fs",
+                    synth.make_protein(138, 0).system_dict(), 1000, 1000),
+                   ("globule: 2,500 atoms compacted to a 20 A radius, no cutoff, BAOAB 300 K, gamma 1/ps, dt 1 fs",
+                    synth.make_protein(2500, 1, radius=synth.compact_radius(2500)).system_dict(), 10000, 20)]
     systems = [p.system_dict() for p in prots]
     sizes = np.array([p.n_atoms for p in prots], np.int64)
     pairs_per_step = float(np.sum(sizes * (sizes - 1) / 2))
@@ -181,6 +237,8 @@ def main():
         "peak": PEAK_FP32,
         "unit": "TFLOP/s",
         "frac": achieved / PEAK_FP32,
+        "peak_unpacked": PEAK_FP32_UNPACKED,
+        "frac_unpacked": achieved / PEAK_FP32_UNPACKED,
         "traffic": traffic,
         "traffic_unit": "bytes per launch (HBM, rocprofv3 FETCH_SIZE + WRITE_SIZE scaled by a 256 MiB copy; "
                         "profiles/hbm_traffic_batch256.md)",
@@ -268,7 +326,16 @@ def main():
         log(f"serving: {len(reqs)} requests in {time.time() - t1:.1f}s, p50 {st['latency_p50_s']:.2f}s "
             f"p99 {st['latency_p99_s']:.2f}s")
 
-    # 8. CPU baseline (bounded sample)
+    # 8. one system alone (the 138-atom peptide, the 2,500-atom globule), GPU and CPU
+    single_out = []
+    for label, sd, gsteps, csteps in singles:
+        t0 = time.time()
+        single_out.append(single_system(sd, gsteps, csteps, label, not a.no_cpu))
+        log(f"single system {single_out[-1]['atoms']} atoms: {single_out[-1]['gpu_ms_per_step']:.4f} ms/step GPU"
+            + (f", {single_out[-1]['cpu_ms_per_step']:.2f} ms/step CPU" if not a.no_cpu else "")
+            + f" ({time.time() - t0:.0f}s)")
+
+    # 9. CPU baseline (bounded sample)
     cpu = None
     if not a.no_cpu:
         t0 = time.time()
@@ -301,6 +368,7 @@ def main():
         "plain_path": plain,
         "serving": serving,
         "large_cutoff": large_out,
+        "single_systems": single_out or None,
         "temperature_K": {"mean": float(np.mean(temp)), "median": float(np.median(temp)), "min": float(np.min(temp)),
                           "max": float(np.max(temp)), "systems_above_1000K": int(np.sum(temp > 1000.0))},
         "safety_cap_events": {"vlimit": int(warn[0]), "force_cap": int(warn[1]), "dx_cap": int(warn[3])},
