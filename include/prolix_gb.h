@@ -178,6 +178,12 @@ int prx_integrator_state(prx_engine* eng, float* positions, float* momenta, floa
 int prx_integrator_kinetic(prx_engine* eng, double* kinetic, void* stream);
 /* Steps taken so far by system s (host value, synchronises the stream). */
 int prx_integrator_step_count(prx_engine* eng, int32_t system, int64_t* steps);
+/* Subtract each system's mass-weighted centre of mass from its positions
+ * (float64 reduction); momenta and stored forces are unchanged. Replaces
+ * recenter_com (batched_simulate.py:58-81) as applied by
+ * apply_com_correction (batched_simulate.py:122-155) during streaming
+ * production (batched_simulate.py:1226-1422). */
+int prx_recenter_com(prx_engine* eng, void* stream);
 /* Safety-cap event counters of system s (atom-steps that hit the velocity,
  * force, constraint and displacement caps; warn_counts of typing.py:558-563).
  * Host out, 4 values; synchronises the stream. */

@@ -77,6 +77,7 @@ _SIGS = {
     "prx_integrator_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                             ctypes.c_void_p]),
     "prx_integrator_kinetic": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "prx_recenter_com": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "prx_integrator_step_count": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64)]),
     "prx_integrator_warnings": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64)]),
     "prx_pool_create": (ctypes.c_int, [ctypes.POINTER(PoolLimits), ctypes.POINTER(ctypes.c_void_p)]),

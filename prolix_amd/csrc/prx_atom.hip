@@ -399,6 +399,28 @@ __global__ __launch_bounds__(kRedThreads) void k_kinetic(DeviceView v, const int
   if (threadIdx.x == 0) out[blockIdx.x] = ke;
 }
 
+// Positions minus the system's mass-weighted centre of mass (masses from the
+// stored 1/m in mom.w), FP64 sums; one block per listed system.
+__global__ __launch_bounds__(kRedThreads) void k_recenter_com(DeviceView v, const int32_t* ranges) {
+  __shared__ double sh[kRedThreads];
+  const int32_t* R = ranges + blockIdx.x * kRangeInts;
+  double sm = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
+  for (int i = threadIdx.x; i < R[3]; i += kRedThreads) {
+    const int g = R[2] + i;
+    const double m = 1.0 / (double)v.mom[g].w;
+    const float4 x = v.pos[g];
+    sm += m; sx += m * x.x; sy += m * x.y; sz += m * x.z;
+  }
+  const double mt = block_sum(sm, sh) + 1e-30;
+  const double cx = block_sum(sx, sh) / mt, cy = block_sum(sy, sh) / mt, cz = block_sum(sz, sh) / mt;
+  for (int i = threadIdx.x; i < R[3]; i += kRedThreads) {
+    const int g = R[2] + i;
+    float4 x = v.pos[g];
+    x.x = (float)(x.x - cx); x.y = (float)(x.y - cy); x.z = (float)(x.z - cz);
+    v.pos[g] = x;
+  }
+}
+
 // Maxwell-Boltzmann momenta, centred per system (jax_md initialize_momenta):
 // p = sqrt(m kT) xi, then p -= mean(p). One block per listed slot.
 __global__ __launch_bounds__(kRedThreads) void k_init_momenta(DeviceView v, const int32_t* slot_list) {
@@ -520,6 +542,9 @@ void launch_energy_reduce(const DeviceView& v, const int32_t* ranges, int n_slot
 }
 void launch_kinetic(const DeviceView& v, const int32_t* ranges, int n_slots, double* out, hipStream_t s) {
   if (n_slots) hipLaunchKernelGGL(k_kinetic, dim3(n_slots), dim3(kRedThreads), 0, s, v, ranges, out);
+}
+void launch_recenter_com(const DeviceView& v, const int32_t* ranges, int n_slots, hipStream_t s) {
+  if (n_slots) hipLaunchKernelGGL(k_recenter_com, dim3(n_slots), dim3(kRedThreads), 0, s, v, ranges);
 }
 void launch_init_momenta(const DeviceView& v, const int32_t* slot_list, int n, hipStream_t s) {
   if (n) hipLaunchKernelGGL(k_init_momenta, dim3(n), dim3(kRedThreads), 0, s, v, slot_list);

@@ -160,6 +160,7 @@ void launch_copy_in_momenta(const DeviceView& v, const float* packed, const int3
 void launch_init_momenta(const DeviceView& v, const int32_t* slot_list, int n_slot_list, hipStream_t s);
 void launch_kinetic(const DeviceView& v, const int32_t* slot_ranges, int n_slots, double* out,
                     hipStream_t s);
+void launch_recenter_com(const DeviceView& v, const int32_t* slot_ranges, int n_slots, hipStream_t s);
 void launch_born_out(const DeviceView& v, float* out, const int32_t* map, int n, hipStream_t s);
 
 }  // namespace prx

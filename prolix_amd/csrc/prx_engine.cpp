@@ -972,6 +972,16 @@ int prx_integrator_kinetic(prx_engine* eng, double* kinetic, void* stream) {
   });
 }
 
+int prx_recenter_com(prx_engine* eng, void* stream) {
+  return guard([&] {
+    need(eng, "prx_recenter_com: bad arguments");
+    if (!eng->initialized) throw StateError("prx_recenter_com before prx_integrator_init");
+    eng->rebuild();
+    prx::launch_recenter_com(eng->v, eng->d_ranges, (int)eng->order.size(), eng->pick(stream));
+    HIPCHECK(hipGetLastError());
+  });
+}
+
 int prx_integrator_step_count(prx_engine* eng, int32_t system, int64_t* steps) {
   return guard([&] {
     need(eng && steps && system >= 0 && system < (int32_t)eng->order.size(), "prx_integrator_step_count: bad arguments");

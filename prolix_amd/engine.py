@@ -144,6 +144,11 @@ class GBEngine:
         N.check(N.lib().prx_integrator_kinetic(self._h, _vp(ke), _stream(t)), "prx_integrator_kinetic")
         return ke
 
+    def recenter_com(self):
+        """Subtract each system's mass-weighted centre of mass from its positions
+        (batched_simulate.py:58-81); momenta and forces are unchanged."""
+        N.check(N.lib().prx_recenter_com(self._h, _stream(self.torch)), "prx_recenter_com")
+
     def step_count(self, system: int = 0) -> int:
         c = ctypes.c_int64()
         N.check(N.lib().prx_integrator_step_count(self._h, int(system), ctypes.byref(c)), "prx_integrator_step_count")

@@ -191,3 +191,108 @@ def engine_states(engine, masses_amu, dt_fs: float, energies: bool = True) -> li
             mass=mk, potential_energy=None if e is None else np.float64(e[k, -1]),
             kinetic_energy=None if ke is None else np.float64(ke[k])))
     return out
+
+
+DEFAULT_RECENTER_EVERY = 100  # steps between centre-of-mass re-centrings (batched_simulate.py:55)
+
+
+def _next_recenter(step: int, every: int) -> int:
+    """Step count after which the next re-centring falls: the step with 0-based
+    index i > 0 and i % every == 0 ends at count i + 1."""
+    j = max(1, (step - 1) // every + 1)
+    return j * every + 1
+
+
+def produce_streaming(engine, n_saves: int, steps_per_save: int, write_fn, *,
+                      recenter_every: int = DEFAULT_RECENTER_EVERY, write_batch_size: int = 1,
+                      start_frame: int = 0, device_offset: int = 0) -> int:
+    """Streaming production on an initialised GBEngine (batched_simulate.py:1226-1422).
+
+    Runs (n_saves - start_frame) * steps_per_save steps of the engine's BAOAB
+    integrator and returns that count. Nothing accumulates on the device:
+
+    - after the step with 0-based index i (counted from this call) where
+      i > 0 and i % recenter_every == 0, every system's positions are
+      re-centred on its mass-weighted centre of mass on the device
+      (apply_com_correction without rotation removal, batched_simulate.py:122-155);
+      recenter_every <= 0 turns this off;
+    - after every steps_per_save steps the positions are copied to pinned host
+      memory behind the step launches, and the host writes frame k - 1 while
+      the GPU runs the steps of frame k;
+    - write_fn(positions (N_s, 3) float32, batch_idx, save_idx) is called per
+      system s, with batch_idx = s + device_offset; with write_batch_size
+      W > 1 frames are grouped, write_fn(frames (W, N_s, 3), batch_idx,
+      start_save_idx), and W must divide n_saves - start_frame.
+
+    Save indices start at start_frame, as in the reference's grouped path.
+    Systems are written without padding atoms.
+    """
+    t = engine.torch
+    remaining = int(n_saves) - int(start_frame)
+    if remaining < 0 or steps_per_save <= 0 or write_batch_size < 1:
+        raise ValueError("need n_saves >= start_frame, steps_per_save > 0 and write_batch_size >= 1")
+    if remaining % write_batch_size:
+        raise ValueError(f"write_batch_size={write_batch_size} must divide the {remaining} remaining saves")
+    offsets = [int(o) for o in engine.offsets]
+    on_device = t.cuda.is_available()
+    host = [t.empty((engine.n_atoms, 3), dtype=t.float32, pin_memory=on_device) for _ in range(2)]
+    groups = [[] for _ in range(engine.n_systems)]
+    step = 0
+
+    def run_to(target):
+        nonlocal step
+        while step < target:
+            nxt = target
+            if recenter_every > 0:
+                nxt = min(nxt, _next_recenter(step, recenter_every))
+            engine.steps(nxt - step)
+            step = nxt
+            if recenter_every > 0 and step > 1 and (step - 1) % recenter_every == 0:
+                engine.recenter_com()
+
+    def flush(frame):
+        ev, buf, save_idx = frame
+        if ev is not None:
+            ev.synchronize()
+        x = buf.numpy()
+        for s in range(engine.n_systems):
+            xs = x[offsets[s]:offsets[s + 1]].copy()
+            if write_batch_size == 1:
+                write_fn(xs, s + device_offset, save_idx)
+                continue
+            groups[s].append(xs)
+            if len(groups[s]) == write_batch_size:
+                write_fn(np.stack(groups[s]), s + device_offset, save_idx - write_batch_size + 1)
+                groups[s] = []
+
+    pending = None
+    for k in range(remaining):
+        run_to((k + 1) * steps_per_save)
+        x = engine.state()[0]
+        buf = host[k % 2]
+        buf.copy_(x, non_blocking=on_device)
+        ev = None
+        if on_device:
+            ev = t.cuda.Event()
+            ev.record()
+        if pending is not None:
+            flush(pending)
+        pending = (ev, buf, start_frame + k)
+    if pending is not None:
+        flush(pending)
+    return step
+
+
+def trajectory_write_fn(writers, dt_fs: float, steps_per_save: int):
+    """A write_fn for produce_streaming that appends SimulationState frames
+    (positions only; velocities zero-filled, as the stream carries positions)
+    to one TrajectoryWriter per batch index."""
+
+    def write(positions, batch_idx, save_idx):
+        frames = positions if positions.ndim == 3 else positions[None]
+        for k, xs in enumerate(frames):
+            step = (save_idx + k + 1) * steps_per_save
+            writers[batch_idx].write(SimulationState(positions=xs, velocities=np.zeros_like(xs),
+                                                     step=np.int64(step), time_ns=np.float64(step * dt_fs * 1e-6)))
+
+    return write

@@ -41,3 +41,74 @@ def test_engine_states_round_trip_through_a_trajectory(tmp_path):
         # 1 amu A^2 / tau^2 = 1 kcal/mol in AKMA units.
         ke_v = 0.5 * float((masses[sl, None] * st.velocities.astype(np.float64) ** 2).sum())
         assert st.kinetic_energy == ke[k] and ke_v == pytest.approx(ke[k], rel=1e-5)
+
+
+def _pair_engine():
+    from prolix_amd.engine import GBEngine, Langevin
+
+    systems = [protein(60, 3), protein(95, 4)]
+    eng = GBEngine(systems)
+    eng.init([s["positions"] for s in systems], [Langevin(1.0, 300.0, 1.0, seed=5), Langevin(1.0, 310.0, 1.0, seed=6)])
+    masses = [np.asarray(s["masses"], np.float64) for s in systems]
+    return eng, masses
+
+
+def _com(x, m):
+    return (m[:, None] * x).sum(0) / m.sum()
+
+
+def test_recenter_com_centres_each_system_and_keeps_momenta():
+    print("[gpu] recenter_com, 2 systems", flush=True)
+    eng, masses = _pair_engine()
+    eng.steps(10)
+    x0, p0, f0 = (a.cpu().numpy() for a in eng.state())
+    eng.recenter_com()
+    x1, p1, f1 = (a.cpu().numpy() for a in eng.state())
+    eng.close()
+    assert np.array_equal(p0, p1) and np.array_equal(f0, f1)
+    for xs0, xs1, m in zip(np.split(x0, [60]), np.split(x1, [60]), masses):
+        assert np.abs(_com(xs1.astype(np.float64), m)).max() < 1e-4
+        assert np.allclose(xs1, xs0 - _com(xs0.astype(np.float64), m), atol=1e-5)
+
+
+def test_streaming_frames_equal_plain_steps():
+    from prolix_amd import io
+
+    print("[gpu] produce_streaming vs steps, no re-centring", flush=True)
+    eng, _ = _pair_engine()
+    got = {}
+    n = io.produce_streaming(eng, 3, 20, lambda x, b, k: got.__setitem__((b, k), x), recenter_every=0)
+    eng.close()
+    ref, _ = _pair_engine()
+    frames = []
+    for _ in range(3):
+        ref.steps(20)
+        frames.append(ref.state()[0].cpu().numpy())
+    ref.close()
+    assert n == 60 and sorted(got) == [(b, k) for b in range(2) for k in range(3)]
+    for k in range(3):
+        assert np.array_equal(got[(0, k)], frames[k][:60]) and np.array_equal(got[(1, k)], frames[k][60:])
+
+
+def test_streaming_recentring_only_translates(tmp_path):
+    from prolix_amd import io
+
+    print("[gpu] produce_streaming with re-centring every 10 steps", flush=True)
+    eng, masses = _pair_engine()
+    writers = [io.TrajectoryWriter(str(tmp_path / f"s{b}.prx")) for b in range(2)]
+    io.produce_streaming(eng, 4, 11, io.trajectory_write_fn(writers, 1.0, 11), recenter_every=10)
+    for w in writers:
+        w.close()
+    eng.close()
+    ref, _ = _pair_engine()
+    ref.steps(44)
+    xr = ref.state()[0].cpu().numpy()
+    ref.close()
+    for b, (sl, m) in enumerate(zip((slice(0, 60), slice(60, 155)), masses)):
+        frames = list(io.read_trajectory(str(tmp_path / f"s{b}.prx")))
+        assert [int(f.step) for f in frames] == [11, 22, 33, 44]
+        # the frame at 11 steps directly follows the re-centring after step index 10
+        assert np.abs(_com(frames[0].positions.astype(np.float64), m)).max() < 1e-4
+        last = frames[-1].positions.astype(np.float64)
+        want = xr[sl].astype(np.float64)
+        assert np.allclose(last - _com(last, m), want - _com(want, m), atol=1e-3)

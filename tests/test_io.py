@@ -136,3 +136,79 @@ def test_truncated_or_foreign_files_raise(tmp_path):
     (tmp_path / "other.prx").write_bytes(b"NOTATRJ\n" + raw[8:])
     with pytest.raises(ValueError, match="not a trajectory"):
         list(io.read_trajectory(str(tmp_path / "other.prx")))
+
+
+class _FakeEngine:
+    """Host stand-in for GBEngine's streaming surface: each step adds a fixed
+    per-system velocity to the positions; re-centring subtracts the mass-weighted
+    centre of mass. Records the call sequence."""
+
+    def __init__(self, sizes, seed=0):
+        import torch
+
+        self.torch = torch
+        self.n_systems = len(sizes)
+        self.offsets = np.concatenate([[0], np.cumsum(sizes)])
+        self.n_atoms = int(self.offsets[-1])
+        rng = np.random.default_rng(seed)
+        self.x = rng.normal(size=(self.n_atoms, 3))
+        self.m = rng.uniform(1, 16, self.n_atoms)
+        self.v = rng.normal(size=(self.n_atoms, 3)) * 0.01
+        self.calls = []
+        self.count = 0
+
+    def steps(self, n):
+        self.calls.append(("steps", n))
+        self.x = self.x + n * self.v
+        self.count += n
+
+    def recenter_com(self):
+        self.calls.append(("recenter", self.count))
+        for s in range(self.n_systems):
+            sl = slice(self.offsets[s], self.offsets[s + 1])
+            self.x[sl] -= (self.m[sl, None] * self.x[sl]).sum(0) / self.m[sl].sum()
+
+    def state(self):
+        x = self.torch.tensor(self.x, dtype=self.torch.float32)
+        return x, None, None
+
+
+def test_streaming_schedule_and_frames():
+    eng = _FakeEngine([5, 9])
+    got = []
+    n = io.produce_streaming(eng, 6, 7, lambda x, b, k: got.append((b, k, x)), recenter_every=10)
+    assert n == 42 and eng.count == 42
+    # re-centrings after the steps with 0-based index 10, 20, 30, 40
+    assert [c for name, c in eng.calls if name == "recenter"] == [11, 21, 31, 41]
+    assert [(b, k) for b, k, _ in got] == [(b, k) for k in range(6) for b in range(2)]
+    # the last frame of system 1: its centre of mass moved one step past the re-centring at 41
+    xs = got[-1][2]
+    sl = slice(5, 14)
+    com = (eng.m[sl, None] * xs).sum(0) / eng.m[sl].sum()
+    drift = (eng.m[sl, None] * eng.v[sl]).sum(0) / eng.m[sl].sum()
+    assert np.allclose(com, drift, atol=1e-5)
+
+
+def test_streaming_grouped_writes_and_resume():
+    eng = _FakeEngine([4])
+    got = []
+    io.produce_streaming(eng, 8, 3, lambda x, b, k: got.append((b, k, x.shape)), recenter_every=0,
+                         write_batch_size=3, start_frame=2)
+    assert got == [(0, 2, (3, 4, 3)), (0, 5, (3, 4, 3))]
+    assert all(name == "steps" for name, _ in eng.calls) and eng.count == 18
+    with pytest.raises(ValueError, match="must divide"):
+        io.produce_streaming(eng, 8, 3, lambda *a: None, write_batch_size=4, start_frame=1)
+
+
+def test_streaming_into_trajectory_files(tmp_path):
+    eng = _FakeEngine([3, 4])
+    paths = [tmp_path / f"s{k}.prx" for k in range(2)]
+    writers = [io.TrajectoryWriter(str(p)) for p in paths]
+    io.produce_streaming(eng, 4, 5, io.trajectory_write_fn(writers, dt_fs=2.0, steps_per_save=5), recenter_every=0)
+    for w in writers:
+        w.close()
+    frames = [list(io.read_trajectory(str(p))) for p in paths]
+    assert [len(f) for f in frames] == [4, 4]
+    assert [int(st.step) for st in frames[1]] == [5, 10, 15, 20]
+    assert frames[1][-1].time_ns == pytest.approx(20 * 2.0e-6)
+    assert np.allclose(frames[1][-1].positions, eng.x[3:7], atol=1e-6)
