@@ -190,14 +190,22 @@ int prx_recenter_com(prx_engine* eng, void* stream);
 int prx_integrator_warnings(prx_engine* eng, int32_t system, int64_t* counts);
 
 /* ---- measurement --------------------------------------------------------
- * Launch each force pass of one step on its own, n_reps times, bracketed by
- * HIP events on the engine stream, at the current positions; ms receives the
- * mean duration per launch of: Born pairs, Born gather, force pairs, force
- * gather, chain pairs, bonded, chain gather (7 doubles). The integrator's
+ * Launch each force pass of one step n_reps times back to back, the run
+ * bracketed by one pair of HIP events on the engine stream, at the current
+ * positions; ms receives the mean duration per launch of: Born pairs, Born
+ * gather, force pairs, force gather, chain pairs, bonded, chain gather
+ * (7 doubles). The integrator's
  * positions and momenta are untouched; its stored forces are recomputed at
  * the same positions.
  * No counterpart in prolix (its timing is wall-clock around jitted calls). */
 int prx_profile_passes(prx_engine* eng, int32_t n_reps, double* ms);
+/* Take n_steps (1..1000) integrator steps through a step graph that holds a
+ * one-lane kernel storing the GPU wall clock before and after every pass; ms
+ * receives each pass's mean duration inside the graph, launch gap included
+ * (same 7 passes and order as prx_profile_passes).
+ * The integrator advances n_steps like prx_integrator_steps.
+ * No counterpart in prolix. */
+int prx_profile_steps(prx_engine* eng, int32_t n_steps, double* ms);
 /* Number of pair work items (64x64 tiles) of the engine's current work list. */
 int prx_engine_work_items(const prx_engine* eng, int64_t* n_items);
 

@@ -89,6 +89,7 @@ _SIGS = {
     "prx_pool_slots": (ctypes.c_int, [ctypes.c_void_p, c_i32p, ctypes.c_int32, c_i32p]),
     "prx_pool_energy": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, c_f64p, ctypes.c_void_p]),
     "prx_profile_passes": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, c_f64p]),
+    "prx_profile_steps": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, c_f64p]),
     "prx_engine_work_items": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]),
     "prx_cmap_coefficients": (ctypes.c_int, [c_f64p, ctypes.c_int32, c_f64p]),
     "prx_plan_tiles": (ctypes.c_int, [ctypes.POINTER(SystemDesc), ctypes.POINTER(ctypes.c_int64),

@@ -543,6 +543,17 @@ void launch_energy_reduce(const DeviceView& v, const int32_t* ranges, int n_slot
 void launch_kinetic(const DeviceView& v, const int32_t* ranges, int n_slots, double* out, hipStream_t s) {
   if (n_slots) hipLaunchKernelGGL(k_kinetic, dim3(n_slots), dim3(kRedThreads), 0, s, v, ranges, out);
 }
+// One timestamp of the GPU's constant-rate wall clock, stored by lane 0 of a
+// one-wave launch; graph dependencies order it between the passes around it.
+__global__ void k_tick(unsigned long long* out) {
+  const unsigned long long t = wall_clock64();
+  if (threadIdx.x == 0) out[threadIdx.x] = t;
+}
+
+void launch_tick(unsigned long long* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, s, out);
+}
+
 void launch_recenter_com(const DeviceView& v, const int32_t* ranges, int n_slots, hipStream_t s) {
   if (n_slots) hipLaunchKernelGGL(k_recenter_com, dim3(n_slots), dim3(kRedThreads), 0, s, v, ranges);
 }

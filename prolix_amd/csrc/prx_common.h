@@ -161,6 +161,7 @@ void launch_init_momenta(const DeviceView& v, const int32_t* slot_list, int n_sl
 void launch_kinetic(const DeviceView& v, const int32_t* slot_ranges, int n_slots, double* out,
                     hipStream_t s);
 void launch_recenter_com(const DeviceView& v, const int32_t* slot_ranges, int n_slots, hipStream_t s);
+void launch_tick(unsigned long long* out, hipStream_t s);
 void launch_born_out(const DeviceView& v, float* out, const int32_t* map, int n, hipStream_t s);
 
 }  // namespace prx

@@ -1112,31 +1112,111 @@ int prx_profile_passes(prx_engine* eng, int32_t n_reps, double* ms) {
       HIPCHECK(hipEventCreate(&ev[k][1]));
       ms[k] = 0.0;
     }
-    for (int r = 0; r < n_reps; ++r) {
-      auto timed = [&](int k, auto&& launch) {
-        HIPCHECK(hipEventRecord(ev[k][0], s));
-        launch();
-        HIPCHECK(hipEventRecord(ev[k][1], s));
-        HIPCHECK(hipEventSynchronize(ev[k][1]));
-        float t = 0.f;
-        HIPCHECK(hipEventElapsedTime(&t, ev[k][0], ev[k][1]));
-        ms[k] += t;
-      };
-      prx::launch_tile_boxes(eng->v, s);
-      timed(0, [&] { prx::launch_born_pairs(eng->v, s); });
-      timed(1, [&] { prx::launch_born_gather(eng->v, s); });
-      timed(2, [&] { prx::launch_force_pairs(eng->v, false, s); });
-      timed(3, [&] { prx::launch_force_gather(eng->v, false, s); });
-      timed(4, [&] { prx::launch_chain_pairs(eng->v, s); });
-      timed(5, [&] { prx::launch_bonded(eng->v, false, s); });
-      timed(6, [&] { prx::launch_chain_gather(eng->v, -1, s); });
-    }
+    // Each pass is launched n_reps times back to back between one pair of
+    // events (every pass is idempotent on unchanged inputs), so the mean is
+    // the kernel's own duration plus the launch gap, comparable with a
+    // kernel trace.
+    auto timed = [&](int k, auto&& launch) {
+      HIPCHECK(hipEventRecord(ev[k][0], s));
+      for (int r = 0; r < n_reps; ++r) launch();
+      HIPCHECK(hipEventRecord(ev[k][1], s));
+      HIPCHECK(hipEventSynchronize(ev[k][1]));
+      float t = 0.f;
+      HIPCHECK(hipEventElapsedTime(&t, ev[k][0], ev[k][1]));
+      ms[k] = t;
+    };
+    prx::launch_tile_boxes(eng->v, s);
+    timed(0, [&] { prx::launch_born_pairs(eng->v, s); });
+    timed(1, [&] { prx::launch_born_gather(eng->v, s); });
+    timed(2, [&] { prx::launch_force_pairs(eng->v, false, s); });
+    timed(3, [&] { prx::launch_force_gather(eng->v, false, s); });
+    timed(4, [&] { prx::launch_chain_pairs(eng->v, s); });
+    timed(5, [&] { prx::launch_bonded(eng->v, false, s); });
+    timed(6, [&] { prx::launch_chain_gather(eng->v, -1, s); });
     for (int k = 0; k < n_pass; ++k) {
       ms[k] /= n_reps;
       hipEventDestroy(ev[k][0]);
       hipEventDestroy(ev[k][1]);
     }
     HIPCHECK(hipGetLastError());
+  });
+}
+
+int prx_profile_steps(prx_engine* eng, int32_t n_steps, double* ms) {
+  return guard([&] {
+    need(eng && ms && n_steps > 0 && n_steps <= 1000, "prx_profile_steps: bad arguments");
+    if (!eng->initialized) throw StateError("prx_profile_steps before prx_integrator_init");
+    eng->rebuild();
+    if (eng->order.empty()) throw std::invalid_argument("prx_profile_steps: no systems");
+    HIPCHECK(hipDeviceSynchronize());
+    constexpr int n_pass = 7;  // Born pairs, Born gather, force pairs, force gather, chain pairs, bonded, chain gather
+    hipStream_t s = eng->stream;
+    prx::DeviceView& v = eng->v;
+    int dev = 0, khz = 0;
+    HIPCHECK(hipGetDevice(&dev));
+    HIPCHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+    if (khz <= 0) throw std::runtime_error("prx_profile_steps: no wall-clock rate");
+    const size_t n_ticks = (size_t)n_steps * n_pass * 2;
+    unsigned long long* ticks = nullptr;
+    hipGraph_t g = nullptr;
+    hipGraphExec_t exec = nullptr;
+    auto cleanup = [&] {
+      if (exec) hipGraphExecDestroy(exec);
+      if (g) hipGraphDestroy(g);
+      if (ticks) hipFree(ticks);
+    };
+    try {
+      HIPCHECK(hipMalloc(&ticks, n_ticks * sizeof(unsigned long long)));
+      auto T = [&](int k, int p, int b) { return ticks + ((size_t)k * n_pass + p) * 2 + b; };
+      // The step graph of prx_integrator_steps with a wall-clock tick before
+      // and after every pass (bonded and its ticks on the side stream).
+      HIPCHECK(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+      try {
+        prx::launch_step_begin(v, s);
+        for (int k = 0; k < n_steps; ++k) {
+          auto timed = [&](int p, hipStream_t st, auto&& launch) {
+            prx::launch_tick(T(k, p, 0), st);
+            launch();
+            prx::launch_tick(T(k, p, 1), st);
+          };
+          HIPCHECK(hipEventRecord(eng->ev_fork, s));
+          HIPCHECK(hipStreamWaitEvent(eng->side, eng->ev_fork, 0));
+          timed(5, eng->side, [&] { prx::launch_bonded(v, false, eng->side); });
+          HIPCHECK(hipEventRecord(eng->ev_join, eng->side));
+          prx::launch_tile_boxes(v, s);
+          timed(0, s, [&] { prx::launch_born_pairs(v, s); });
+          timed(1, s, [&] { prx::launch_born_gather(v, s); });
+          timed(2, s, [&] { prx::launch_force_pairs(v, false, s); });
+          timed(3, s, [&] { prx::launch_force_gather(v, false, s); });
+          timed(4, s, [&] { prx::launch_chain_pairs(v, s); });
+          HIPCHECK(hipStreamWaitEvent(s, eng->ev_join, 0));
+          timed(6, s, [&] { prx::launch_chain_gather(v, k + 1 == n_steps ? 2 : 1, s); });
+        }
+        HIPCHECK(hipGetLastError());
+      } catch (...) {
+        hipGraph_t partial = nullptr;
+        if (hipStreamEndCapture(s, &partial) == hipSuccess && partial) hipGraphDestroy(partial);
+        throw;
+      }
+      HIPCHECK(hipStreamEndCapture(s, &g));
+      HIPCHECK(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
+      HIPCHECK(hipGraphLaunch(exec, s));
+      HIPCHECK(hipStreamSynchronize(s));
+      std::vector<unsigned long long> h(n_ticks);
+      HIPCHECK(hipMemcpy(h.data(), ticks, n_ticks * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+      for (int p = 0; p < n_pass; ++p) {
+        double sum = 0.0;
+        for (int k = 0; k < n_steps; ++k) {
+          const size_t i = ((size_t)k * n_pass + p) * 2;
+          sum += (double)(h[i + 1] - h[i]) / (double)khz;  // ticks at khz kHz -> ms
+        }
+        ms[p] = sum / n_steps;
+      }
+    } catch (...) {
+      cleanup();
+      throw;
+    }
+    cleanup();
   });
 }
 

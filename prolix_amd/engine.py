@@ -162,6 +162,13 @@ class GBEngine:
         N.check(N.lib().prx_profile_passes(self._h, int(n_reps), ms), "prx_profile_passes")
         return dict(zip(self.PASSES, list(ms)))
 
+    def profile_steps(self, n_steps: int = 20) -> dict:
+        """Mean HIP-event duration (ms) of each force pass inside the step graph,
+        over n_steps real integrator steps (the integrator advances)."""
+        ms = (ctypes.c_double * 7)()
+        N.check(N.lib().prx_profile_steps(self._h, int(n_steps), ms), "prx_profile_steps")
+        return dict(zip(self.PASSES, list(ms)))
+
     def work_items(self) -> int:
         n = ctypes.c_int64()
         N.check(N.lib().prx_engine_work_items(self._h, ctypes.byref(n)), "prx_engine_work_items")

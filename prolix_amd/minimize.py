@@ -15,18 +15,29 @@ import numpy as np
 FRACTIONS = (1.0, 0.5, 0.25, 0.1)
 
 
+def _lengths(engine):
+    """Per-system atom counts as a device tensor: the packed layout keeps each
+    system's atoms contiguous, so per-system reductions are segment reductions
+    (no contended atomics, a fixed summation order)."""
+    cached = getattr(engine, "_segment_lengths", None)
+    if cached is None:
+        cached = engine.torch.as_tensor(np.asarray(engine.sizes, np.int64), device=engine.device)
+        engine._segment_lengths = cached
+    return cached
+
+
 def minimize(engine, positions, steps: int = 200, dt_max: float = 4e-3, max_displacement: float = 0.1):
     """Returns the relaxed packed positions (device float32 tensor)."""
     t = engine.torch
     x = engine._pos_tensor(positions).clone()
     sid = t.as_tensor(np.repeat(np.arange(engine.n_systems), engine.sizes), device=x.device)
+    lengths = _lengths(engine)
     total = 11  # index of the total energy
     for _ in range(int(steps)):
         e, f, _ = engine.energy_forces(x, energies=True)
         e0 = e[:, total]
         fmag = t.linalg.vector_norm(f, dim=1)
-        fmax = t.zeros(engine.n_systems, dtype=f.dtype, device=f.device).scatter_reduce(
-            0, sid, fmag, reduce="amax", include_self=True) + 1e-8
+        fmax = t.segment_reduce(fmag, "max", lengths=lengths) + 1e-8
         dt = t.clamp(max_displacement / fmax, max=dt_max)
         best_x = x
         best_e = e0
@@ -102,8 +113,10 @@ def fire_step(engine, state, masses, dt_max, force_cap=None, disp_cap=0.1, **kw)
     f_new = f_new.clone()
     p = p + 0.5 * dt * f_new
 
+    lengths = _lengths(engine)
+
     def per_system(v):
-        return t.zeros(ns, dtype=t.float64, device=v.device).index_add_(0, sid, v.to(t.float64))
+        return t.segment_reduce(v.to(t.float64), "sum", lengths=lengths)
 
     f_norm = t.sqrt(per_system((f_new * f_new).sum(1)) + 1e-6)
     p_norm = t.sqrt(per_system((p * p).sum(1)))

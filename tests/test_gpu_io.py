@@ -112,3 +112,20 @@ def test_streaming_recentring_only_translates(tmp_path):
         last = frames[-1].positions.astype(np.float64)
         want = xr[sl].astype(np.float64)
         assert np.allclose(last - _com(last, m), want - _com(want, m), atol=1e-3)
+
+
+def test_profile_steps_advances_like_steps():
+    print("[gpu] profile_steps vs steps", flush=True)
+    a, _ = _pair_engine()
+    b, _ = _pair_engine()
+    ms = a.profile_steps(7)
+    b.steps(7)
+    xa, pa, _ = (t.cpu().numpy() for t in a.state())
+    xb, pb, _ = (t.cpu().numpy() for t in b.state())
+    assert a.step_count(1) == b.step_count(1) == 7
+    a.close()
+    b.close()
+    assert np.array_equal(xa, xb) and np.array_equal(pa, pb)
+    assert set(ms) == {"born_pairs", "born_gather", "force_pairs", "force_gather", "chain_pairs", "bonded",
+                       "chain_gather"}
+    assert all(0.0 < v < 100.0 for v in ms.values())
