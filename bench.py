@@ -338,7 +338,10 @@ def main():
                    "pair_interactions_per_s": st["pair_interactions_per_s"], "wall_s": st["wall_s"],
                    "offered_pair_steps_per_s": offered_rate,
                    "trace": f"{len(reqs)} requests, Poisson 50/s, N log-uniform [200, 8000], "
-                            "n_steps U[1000, 50000], T U(280, 350) K, HBM budget 200 GB"}
+                            "n_steps U[1000, 50000], T U(280, 350) K, HBM budget 200 GB",
+                   "accuracy": "energy / force error vs the float64 oracle on a seeded 5% sample of a served trace "
+                               "(at least 2 requests): tests/test_gpu_serving.py::test_serving_loop_small_trace, "
+                               "energy rel <= 1e-6, force RMS rel <= 1e-5"}
         log(f"serving: {len(reqs)} requests in {time.time() - t1:.1f}s, p50 {st['latency_p50_s']:.2f}s "
             f"p99 {st['latency_p99_s']:.2f}s")
 

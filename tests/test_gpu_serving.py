@@ -122,17 +122,26 @@ def test_serving_loop_small_trace():
     systems = [protein(int(n), 80 + k, radius=synth.compact_radius(int(n)))
                for k, n in enumerate((220, 800, 350, 1500, 260, 640))]
     reqs = poisson_trace(systems, rate_per_s=50.0, steps_range=(100, 400), seed=3)
+    # a seeded 5% sample of the requests, at least two, checked against the
+    # float64 oracle at the served (float32) final positions
+    n_sample = max(2, int(np.ceil(0.05 * len(reqs))))
+    sample = sorted(np.random.default_rng(4).choice(len(reqs), size=n_sample, replace=False).tolist())
     print("[gpu] serve", flush=True)
-    results, stats = serve(reqs, chunk_steps=50, sample_energies=(0, 3))
+    results, stats = serve(reqs, chunk_steps=50, sample_energies=sample)
     assert len(results) == len(reqs)
     for r, q in zip(results, reqs):
         assert r.n_steps == q.n_steps
         assert np.isfinite(r.positions).all()
     assert stats["latency_p99_s"] >= stats["latency_p50_s"] > 0
-    e0 = results[0].energies
-    sysd = dict(reqs[0].system)
-    terms = O.energy_terms(sysd, results[0].positions.astype(np.float64))
-    assert e0[-1] == pytest.approx(terms["total"], rel=1e-5, abs=1e-3)
+    for k in sample:
+        sysd = dict(reqs[k].system)
+        x = results[k].positions.astype(np.float64)
+        terms = O.energy_terms(sysd, x)
+        assert results[k].energies[-1] == pytest.approx(terms["total"], rel=1e-6, abs=1e-3)
+        f_ref = O.forces(sysd, x)
+        keep = np.linalg.norm(f_ref, axis=1) < 9e3  # below the batched step's force cap
+        err = results[k].forces[keep].astype(np.float64) - f_ref[keep]
+        assert np.sqrt((err ** 2).sum() / (f_ref[keep] ** 2).sum()) <= 1e-5
 
 
 def test_make_energy_fn_neighbor_argument_and_gb_cutoff():
