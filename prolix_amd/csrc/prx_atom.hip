@@ -212,8 +212,8 @@ __device__ __forceinline__ float3 bonded_force(const DeviceView& v, int g, int q
 }
 
 // BAOAB first half (settle.py:2064-2087): B(F_n) A O A for atom g of slot s.
-__device__ __forceinline__ void begin_step(const DeviceView& v, const SlotConst& S, int s, int g, float4& p,
-                                           uint32_t step) {
+__device__ __forceinline__ float4 begin_step(const DeviceView& v, const SlotConst& S, int s, int g, float4& p,
+                                             uint32_t step) {
   const float4 f = v.force[g];
   p.x = fmaf(S.half_dt, f.x, p.x);
   p.y = fmaf(S.half_dt, f.y, p.y);
@@ -251,21 +251,15 @@ __device__ __forceinline__ void begin_step(const DeviceView& v, const SlotConst&
     }
   }
   v.pos[g] = x;
+  return x;
 }
 
-// Total force = chain partials + pair force + bonded records; then
-//   mode -1: store it; 2: B(F) and count the step; 1: B(F), then B A O A.
-__global__ __launch_bounds__(256) void k_chain_gather(DeviceView v, int mode) {
-  if ((int)blockIdx.x >= v.n_tiles) return;
-  const TileCtx c = tile_ctx(v, blockIdx.x);
-  double4 part = gather4(v, c);
-  float3 fb = c.valid ? bonded_force(v, c.g, c.q) : make_float3(0.f, 0.f, 0.f);
-  // the bonded share travels in the same combine (its float sum is exact in FP64)
-  __shared__ float4 shb[kGW][kTile];
-  shb[c.q][c.lane] = make_float4(fb.x, fb.y, fb.z, 0.0f);
-  const double4 a = combine4(part, c);
-  if (c.q != 0 || !c.valid) return;
-  fb = make_float3(0.f, 0.f, 0.f);
+// Total force = chain partials + pair force + bonded records for atom c.g;
+// then mode -1: store it; 2: B(F) and count the step; 1: B(F), then B A O A.
+// Returns the atom's position after the call.
+__device__ __forceinline__ float4 finish_atom(const DeviceView& v, const TileCtx& c, const double4& a,
+                                              const float4 (*shb)[kTile], int mode) {
+  float3 fb = make_float3(0.f, 0.f, 0.f);
 #pragma unroll
   for (int k = 0; k < kGW; ++k) {
     const float4 y = shb[k][c.lane];
@@ -287,15 +281,53 @@ __global__ __launch_bounds__(256) void k_chain_gather(DeviceView v, int mode) {
     }
   }
   v.force[c.g] = make_float4(f.x, f.y, f.z, 0.0f);
-  if (mode < 0 || !S.active) return;
+  if (mode < 0 || !S.active) return v.pos[c.g];
   float4 p = v.mom[c.g];  // w = 1/m
   p.x = fmaf(S.half_dt, f.x, p.x);
   p.y = fmaf(S.half_dt, f.y, p.y);
   p.z = fmaf(S.half_dt, f.z, p.z);
   const uint32_t step = v.atom_step[c.g] + 1u;  // each lane owns its atom's counter
   v.atom_step[c.g] = step;
-  if (mode == 1) begin_step(v, S, c.slot, c.g, p, step);
+  float4 x = mode == 1 ? begin_step(v, S, c.slot, c.g, p, step) : v.pos[c.g];
   v.mom[c.g] = p;
+  return x;
+}
+
+// Per tile (one block): the chain partials, bonded records and pair force
+// finish each atom's force (finish_atom). In mode 1 the tile's atoms have
+// moved to the next step's positions, and wave 0 also writes the tile's box
+// for that step (what k_tile_boxes computes), so the step graph needs no box
+// kernel after its first step.
+__global__ __launch_bounds__(256) void k_chain_gather(DeviceView v, int mode) {
+  if ((int)blockIdx.x >= v.n_tiles) return;
+  const TileCtx c = tile_ctx(v, blockIdx.x);
+  double4 part = gather4(v, c);
+  float3 fb = c.valid ? bonded_force(v, c.g, c.q) : make_float3(0.f, 0.f, 0.f);
+  // the bonded share travels in the same combine (its float sum is exact in FP64)
+  __shared__ float4 shb[kGW][kTile];
+  shb[c.q][c.lane] = make_float4(fb.x, fb.y, fb.z, 0.0f);
+  const double4 a = combine4(part, c);
+  if (c.q != 0) return;
+  float4 x = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (c.valid) x = finish_atom(v, c, a, shb, mode);
+  if (mode != 1 || !v.slots[c.slot].active) return;  // uniform over the tile
+  const int last = v.tiles[blockIdx.x].y - 1;
+  const float xl = __shfl(x.x, last), yl = __shfl(x.y, last), zl = __shfl(x.z, last);
+  if (!c.valid) x = make_float4(xl, yl, zl, 0.f);
+  const float4 b = v.born4[c.g - c.lane + min(c.lane, last)];
+  float lx = x.x, ly = x.y, lz = x.z, hx = x.x, hy = x.y, hz = x.z, mr = b.x, ms = b.y;
+  for (int off = 32; off > 0; off >>= 1) {
+    lx = fminf(lx, __shfl_xor(lx, off));
+    ly = fminf(ly, __shfl_xor(ly, off));
+    lz = fminf(lz, __shfl_xor(lz, off));
+    hx = fmaxf(hx, __shfl_xor(hx, off));
+    hy = fmaxf(hy, __shfl_xor(hy, off));
+    hz = fmaxf(hz, __shfl_xor(hz, off));
+    mr = fmaxf(mr, __shfl_xor(mr, off));
+    ms = fmaxf(ms, __shfl_xor(ms, off));
+  }
+  if (c.lane == 0) v.tbox[2 * blockIdx.x] = make_float4(lx, ly, lz, mr);
+  if (c.lane == 1) v.tbox[2 * blockIdx.x + 1] = make_float4(hx, hy, hz, ms);
 }
 
 // B A O A from the stored force, for the first step of a launch.

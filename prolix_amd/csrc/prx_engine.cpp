@@ -35,9 +35,15 @@ struct StateError : std::runtime_error {
   using std::runtime_error::runtime_error;
 };
 
-// Bonded terms on a side stream beside the pair passes (1) or in line (0).
+// Bonded terms on a side stream beside the pair passes (1) or in line (0),
+// from this many live atoms on: below it the fork's extra graph nodes cost
+// more than the bonded kernel they hide (measured: 8% slower steps for one
+// 138- or 2,500-atom system, 1.2% faster for the 425,157-atom mix).
 #ifndef PRX_FORK_BONDED
 #define PRX_FORK_BONDED 1
+#endif
+#ifndef PRX_FORK_MIN_ATOMS
+#define PRX_FORK_MIN_ATOMS 100000
 #endif
 
 #define HIPCHECK(x)                                                                          \
@@ -655,6 +661,7 @@ struct prx_engine {
   }
 
   int32_t total_atoms() const { return offsets.back(); }
+  bool fork_bonded() const { return PRX_FORK_BONDED && total_atoms() >= PRX_FORK_MIN_ATOMS; }
 
   // ---- evaluation -------------------------------------------------------
   // Everything of one force evaluation except the final chain gather.
@@ -663,25 +670,28 @@ struct prx_engine {
   // and the energy reduction read: with PRX_FORK_BONDED they run on the side
   // stream beside the pair passes and join before this returns (in a
   // captured graph, a parallel branch).
-  void force_passes(bool energy, hipStream_t s) {
-#if PRX_FORK_BONDED
-    HIPCHECK(hipEventRecord(ev_fork, s));
-    HIPCHECK(hipStreamWaitEvent(side, ev_fork, 0));
-    prx::launch_bonded(v, energy, side);
-    HIPCHECK(hipEventRecord(ev_join, side));
-#endif
-    prx::launch_tile_boxes(v, s);
+  // boxes = false when the previous chain gather (mode 1) already wrote the
+  // tile boxes of the current positions.
+  void force_passes(bool energy, hipStream_t s, bool boxes = true) {
+    const bool fork = fork_bonded();
+    if (fork) {
+      HIPCHECK(hipEventRecord(ev_fork, s));
+      HIPCHECK(hipStreamWaitEvent(side, ev_fork, 0));
+      prx::launch_bonded(v, energy, side);
+      HIPCHECK(hipEventRecord(ev_join, side));
+    }
+    if (boxes) prx::launch_tile_boxes(v, s);
     prx::launch_born_pairs(v, s);
     prx::launch_born_gather(v, s);
     prx::launch_force_pairs(v, energy, s);
     prx::launch_force_gather(v, energy, s);
-#if PRX_FORK_BONDED
-    prx::launch_chain_pairs(v, s);
-    HIPCHECK(hipStreamWaitEvent(s, ev_join, 0));
-#else
-    prx::launch_bonded(v, energy, s);
-    prx::launch_chain_pairs(v, s);
-#endif
+    if (fork) {
+      prx::launch_chain_pairs(v, s);
+      HIPCHECK(hipStreamWaitEvent(s, ev_join, 0));
+    } else {
+      prx::launch_bonded(v, energy, s);
+      prx::launch_chain_pairs(v, s);
+    }
   }
 
   void evaluate(bool energy, hipStream_t s) {
@@ -703,7 +713,8 @@ struct prx_engine {
     try {
       prx::launch_step_begin(v, stream);
       for (int k = 0; k < n_steps; ++k) {
-        force_passes(false, stream);
+        // after the first step, the previous chain gather wrote the boxes
+        force_passes(false, stream, k == 0);
         prx::launch_chain_gather(v, k + 1 == n_steps ? 2 : 1, stream);
       }
       HIPCHECK(hipGetLastError());
@@ -1179,17 +1190,21 @@ int prx_profile_steps(prx_engine* eng, int32_t n_steps, double* ms) {
             launch();
             prx::launch_tick(T(k, p, 1), st);
           };
-          HIPCHECK(hipEventRecord(eng->ev_fork, s));
-          HIPCHECK(hipStreamWaitEvent(eng->side, eng->ev_fork, 0));
-          timed(5, eng->side, [&] { prx::launch_bonded(v, false, eng->side); });
-          HIPCHECK(hipEventRecord(eng->ev_join, eng->side));
-          prx::launch_tile_boxes(v, s);
+          const bool fork = eng->fork_bonded();
+          if (fork) {
+            HIPCHECK(hipEventRecord(eng->ev_fork, s));
+            HIPCHECK(hipStreamWaitEvent(eng->side, eng->ev_fork, 0));
+            timed(5, eng->side, [&] { prx::launch_bonded(v, false, eng->side); });
+            HIPCHECK(hipEventRecord(eng->ev_join, eng->side));
+          }
+          if (k == 0) prx::launch_tile_boxes(v, s);
           timed(0, s, [&] { prx::launch_born_pairs(v, s); });
           timed(1, s, [&] { prx::launch_born_gather(v, s); });
           timed(2, s, [&] { prx::launch_force_pairs(v, false, s); });
           timed(3, s, [&] { prx::launch_force_gather(v, false, s); });
+          if (!fork) timed(5, s, [&] { prx::launch_bonded(v, false, s); });
           timed(4, s, [&] { prx::launch_chain_pairs(v, s); });
-          HIPCHECK(hipStreamWaitEvent(s, eng->ev_join, 0));
+          if (fork) HIPCHECK(hipStreamWaitEvent(s, eng->ev_join, 0));
           timed(6, s, [&] { prx::launch_chain_gather(v, k + 1 == n_steps ? 2 : 1, s); });
         }
         HIPCHECK(hipGetLastError());

@@ -256,3 +256,25 @@ def test_safety_caps_bound_a_clash():
     assert warn[1] > 0  # force-cap events
     assert steps.max() <= 0.5 + 1e-5
     assert np.all(np.isfinite(xs[-1])) and np.all(np.linalg.norm(f, axis=1) <= 1e4 * (1 + 1e-6))
+
+
+@pytest.mark.parametrize("cutoff", [0.0, 12.0])
+def test_graph_forces_equal_eager_forces_at_same_positions(cutoff):
+    """Inside a step graph the tile boxes after the first step come from the
+    previous chain gather; an eager evaluation recomputes them. Both must
+    classify every item alike, so the forces the graph stored at the final
+    positions equal an eager evaluation there bit for bit."""
+    from prolix_amd.engine import GBEngine, Langevin
+
+    print(f"[gpu] graph vs eager forces, cutoff {cutoff}", flush=True)
+    systems = [protein(int(n), 40 + k, radius=synth.compact_radius(int(n))) for k, n in enumerate((700, 1300, 2600))]
+    eng = GBEngine(systems, cutoff=cutoff, gb_cutoff=cutoff > 0)
+    eng.init([s["positions"] for s in systems], [Langevin(1.0, 300.0, 1.0, seed=9 + k, safety_caps=True)
+                                                 for k in range(3)])
+    eng.steps(7)
+    x, _, f = eng.state()
+    _, f2, _ = eng.energy_forces(x, energies=False)
+    f, f2 = f.cpu().numpy(), f2.cpu().numpy()
+    eng.close()
+    assert np.isfinite(f).all()
+    assert np.array_equal(f, f2)
