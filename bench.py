@@ -156,7 +156,8 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--n-systems", type=int, default=256)
-    ap.add_argument("--min-steps", type=int, default=500)
+    ap.add_argument("--min-steps", type=int, default=2000,
+                    help="steepest-descent steps before dynamics (mix and the cutoff system)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=32)
@@ -303,7 +304,7 @@ def main():
         ls = large.system_dict()
         rc = 20.0
         e2 = GBEngine(ls, cutoff=rc, gb_cutoff=True)
-        xl = minimize(e2, ls["positions"], steps=100)
+        xl = minimize(e2, ls["positions"], steps=a.min_steps)
         e2.init(xl, Langevin(DT_FS, 300.0, 1.0, seed=77, safety_caps=True))
         e2.steps(20)
         torch.cuda.synchronize()
