@@ -90,8 +90,25 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 #define PRX_PRAGMA(x) _Pragma(#x)
 #define PRX_UNROLL(n) PRX_PRAGMA(unroll n)
 
+// Work item of this wave. Workgroups are dispatched to the 8 XCDs round-robin
+// (block b to XCD b mod 8), and each XCD has its own L2. With PRX_XCD_MAP the
+// blocks are renumbered so that XCD x runs one contiguous run of items
+// (consecutive items share their i tile and neighbouring j tiles, which then
+// hit in that XCD's L2): block b = 8k + x becomes x * per + min(x, rem) + k,
+// a bijection on [0, n_blocks) with per = n_blocks / 8, rem = n_blocks % 8.
+#ifndef PRX_XCD_MAP
+#define PRX_XCD_MAP 1
+#endif
+constexpr int kXcds = 8;
+
 __device__ __forceinline__ int wave_id() {
-  return __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)));
+  int b = (int)blockIdx.x;
+#if PRX_XCD_MAP
+  const int nb = (int)gridDim.x, per = nb / kXcds, rem = nb % kXcds;
+  const int x = b % kXcds, k = b / kXcds;
+  b = x * per + min(x, rem) + k;
+#endif
+  return __builtin_amdgcn_readfirstlane((int)(b * (blockDim.x >> 6) + (threadIdx.x >> 6)));
 }
 
 // ---------------------------------------------------------------------------
