@@ -82,7 +82,7 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 #define PRX_UNROLL_FORCE 4
 #endif
 #ifndef PRX_UNROLL_CHAIN
-#define PRX_UNROLL_CHAIN 4
+#define PRX_UNROLL_CHAIN 8
 #endif
 #ifndef PRX_LDS_ACC
 #define PRX_LDS_ACC 0
