@@ -126,6 +126,17 @@ __device__ __forceinline__ double gather1(const DeviceView& v, const TileCtx& c)
   return acc;
 }
 
+// Chain partials: three float arrays of 64 per (item, side).
+__device__ __forceinline__ double4 gather3(const DeviceView& v, const TileCtx& c) {
+  const float* pc = reinterpret_cast<const float*>(v.part);
+  double4 acc = make_double4(0.0, 0.0, 0.0, 0.0);
+  for_tile_items(v, c, [&](int w, int side) {
+    const float* q = pc + ((size_t)w * 2 + side) * (3 * kTile) + c.lane;
+    acc.x += q[0]; acc.y += q[kTile]; acc.z += q[2 * kTile];
+  });
+  return acc;
+}
+
 __device__ __forceinline__ double4 gather4(const DeviceView& v, const TileCtx& c) {
   double4 acc = make_double4(0.0, 0.0, 0.0, 0.0);
   for_tile_items(v, c, [&](int w, int side) {
@@ -301,7 +312,7 @@ __device__ __forceinline__ float4 finish_atom(const DeviceView& v, const TileCtx
 __global__ __launch_bounds__(256) void k_chain_gather(DeviceView v, int mode) {
   if ((int)blockIdx.x >= v.n_tiles) return;
   const TileCtx c = tile_ctx(v, blockIdx.x);
-  double4 part = gather4(v, c);
+  double4 part = gather3(v, c);
   float3 fb = c.valid ? bonded_force(v, c.g, c.q) : make_float3(0.f, 0.f, 0.f);
   // the bonded share travels in the same combine (its float sum is exact in FP64)
   __shared__ float4 shb[kGW][kTile];

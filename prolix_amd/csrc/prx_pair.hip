@@ -48,6 +48,16 @@ __device__ __forceinline__ size_t part_slot(int w, int side, int lane) {
   return ((size_t)w * 2 + side) * kTile + lane;
 }
 
+// Chain partials carry three components: stored as three float arrays of 64
+// per (item, side), each written and read whole by a wave.
+__device__ __forceinline__ void put_part3(const DeviceView& v, int w, int side, int lane, float x, float y,
+                                          float z) {
+  float* pc = reinterpret_cast<float*>(v.part) + ((size_t)w * 2 + side) * (3 * kTile);
+  pc[lane] = x;
+  pc[kTile + lane] = y;
+  pc[2 * kTile + lane] = z;
+}
+
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ float flog2(float x) { return __builtin_amdgcn_logf(x); }
@@ -631,16 +641,16 @@ __device__ __forceinline__ void chain_tile(const DeviceView& v, const PairItem& 
     }
 #endif
   }
-  v.part[part_slot(w, 0, lane)] = make_float4(fxi, fyi, fzi, 0.0f);
+  put_part3(v, w, 0, lane, fxi, fyi, fzi);
 #if PRX_LDS_ACC
   wave_lds_sync();
   {
     const float4 a0 = L.acc[lane], a1 = L.acc[lane + 64];
-    v.part[part_slot(w, 1, lane)] = make_float4(a0.x + a1.x, a0.y + a1.y, a0.z + a1.z, 0.0f);
+    put_part3(v, w, 1, lane, a0.x + a1.x, a0.y + a1.y, a0.z + a1.z);
   }
 #else
   const int home = (lane + kend) & 63;
-  v.part[part_slot(w, 1, home)] = make_float4(fxj, fyj, fzj, 0.0f);
+  put_part3(v, w, 1, home, fxj, fyj, fzj);
 #endif
 }
 
