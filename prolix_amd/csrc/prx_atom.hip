@@ -209,15 +209,30 @@ __global__ __launch_bounds__(256) void k_force_gather(DeviceView v) {
   }
 }
 
-// Bonded records q, q + 4, ... of atom g (wave q's share), in list order.
+// Bonded records q, q + 4, ... of atom g (wave q's share), added in list
+// order. Four records per round: their list indices, then the records, are
+// loaded before any is added, so a lane has up to four loads in flight.
 __device__ __forceinline__ float3 bonded_force(const DeviceView& v, int g, int q) {
   const int2 lr = v.atom_rec[g];
   float bx = 0.0f, by = 0.0f, bz = 0.0f;
-  for (int k = q; k < lr.y; k += kGW) {
-    const float4 r = v.rec[v.rec_list[lr.x + k]];
-    bx += r.x;
-    by += r.y;
-    bz += r.z;
+  constexpr int kBatch = 4;
+  for (int k = q; k < lr.y; k += kBatch * kGW) {
+    int idx[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      const int kk = k + u * kGW;
+      idx[u] = kk < lr.y ? v.rec_list[lr.x + kk] : -1;
+    }
+    float4 r[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) r[u] = idx[u] >= 0 ? v.rec[idx[u]] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      if (idx[u] < 0) break;
+      bx += r[u].x;
+      by += r[u].y;
+      bz += r[u].z;
+    }
   }
   return make_float3(bx, by, bz);
 }
