@@ -118,11 +118,49 @@ __device__ __forceinline__ void for_tile_items(const DeviceView& v, const TileCt
   }
 }
 
+// Like for_tile_items, but with every item's partial loaded by load(item,
+// side) and added by add(value): systems without far-item skipping load four
+// items' partials before adding them, in the same item order, so each lane
+// keeps up to four loads in flight.
+template <class T, class Load, class Add>
+__device__ __forceinline__ void gather_items(const DeviceView& v, const TileCtx& c, Load&& load, Add&& add) {
+  if (c.skip_far) {
+    for_tile_items(v, c, [&](int w, int side) { add(load(w, side)); });
+    return;
+  }
+  constexpr int kBatch = 4;
+  const int r0 = row_start(c, c.T);
+  const int n_row = max(0, (c.nT - c.T - c.q + kGW - 1) / kGW);
+  const int n_col = max(0, (c.T + 1 - c.q + kGW - 1) / kGW);
+  for (int n = 0; n < n_row; n += kBatch) {
+    T val[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u)
+      if (n + u < n_row) val[u] = load(r0 + c.q + kGW * (n + u), 0);
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u)
+      if (n + u < n_row) add(val[u]);
+  }
+  for (int n = 0; n < n_col; n += kBatch) {
+    T val[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u)
+      if (n + u < n_col) {
+        const int I = c.q + kGW * (n + u);
+        val[u] = load(row_start(c, I) + (c.T - I), 1);
+      }
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u)
+      if (n + u < n_col) add(val[u]);
+  }
+}
+
 // Scalar partials (Born sums) of this wave's items for the lane's atom.
 __device__ __forceinline__ double gather1(const DeviceView& v, const TileCtx& c) {
   const float* pb = reinterpret_cast<const float*>(v.part);
   double acc = 0.0;
-  for_tile_items(v, c, [&](int w, int side) { acc += (double)pb[pslot(w, side, c.lane)]; });
+  gather_items<float>(v, c, [&](int w, int side) { return pb[pslot(w, side, c.lane)]; },
+                      [&](float x) { acc += (double)x; });
   return acc;
 }
 
@@ -130,19 +168,20 @@ __device__ __forceinline__ double gather1(const DeviceView& v, const TileCtx& c)
 __device__ __forceinline__ double4 gather3(const DeviceView& v, const TileCtx& c) {
   const float* pc = reinterpret_cast<const float*>(v.part);
   double4 acc = make_double4(0.0, 0.0, 0.0, 0.0);
-  for_tile_items(v, c, [&](int w, int side) {
-    const float* q = pc + ((size_t)w * 2 + side) * (3 * kTile) + c.lane;
-    acc.x += q[0]; acc.y += q[kTile]; acc.z += q[2 * kTile];
-  });
+  gather_items<float3>(
+      v, c,
+      [&](int w, int side) {
+        const float* q = pc + ((size_t)w * 2 + side) * (3 * kTile) + c.lane;
+        return make_float3(q[0], q[kTile], q[2 * kTile]);
+      },
+      [&](float3 p) { acc.x += p.x; acc.y += p.y; acc.z += p.z; });
   return acc;
 }
 
 __device__ __forceinline__ double4 gather4(const DeviceView& v, const TileCtx& c) {
   double4 acc = make_double4(0.0, 0.0, 0.0, 0.0);
-  for_tile_items(v, c, [&](int w, int side) {
-    const float4 p = v.part[pslot(w, side, c.lane)];
-    acc.x += p.x; acc.y += p.y; acc.z += p.z; acc.w += p.w;
-  });
+  gather_items<float4>(v, c, [&](int w, int side) { return v.part[pslot(w, side, c.lane)]; },
+                       [&](float4 p) { acc.x += p.x; acc.y += p.y; acc.z += p.z; acc.w += p.w; });
   return acc;
 }
 
