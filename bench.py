@@ -172,7 +172,7 @@ def main():
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {a.gpus}): this build has no multi-GPU path; "
-              "independent systems would shard by whole systems (see DESIGN.md).", file=sys.stderr)
+              "independent systems would shard by whole systems, one engine per GPU.", file=sys.stderr)
         sys.exit(2)
 
     # 1. synthetic inputs, generated before the process touches the GPU (forked pool)
