@@ -171,7 +171,7 @@ def main():
     ap.add_argument("--out", default=None)
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py runs on one GPU only (got --gpus {a.gpus}): this build has no multi-GPU path; "
+        print(f"bench.py runs on one GPU only (got --gpus {a.gpus}): there is no multi-GPU path; "
               "independent systems would shard by whole systems, one engine per GPU.", file=sys.stderr)
         sys.exit(2)
 
