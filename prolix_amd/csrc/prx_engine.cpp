@@ -664,12 +664,12 @@ struct prx_engine {
   bool fork_bonded() const { return PRX_FORK_BONDED && total_atoms() >= PRX_FORK_MIN_ATOMS; }
 
   // ---- evaluation -------------------------------------------------------
-  // Everything of one force evaluation except the final chain gather.
-  // Everything up to the chain gather. The bonded terms read only positions
-  // and write their records and term energies, which only the chain gather
-  // and the energy reduction read: with PRX_FORK_BONDED they run on the side
-  // stream beside the pair passes and join before this returns (in a
-  // captured graph, a parallel branch).
+  // Everything of one force evaluation up to the chain gather. The bonded
+  // terms read only positions and write their records and term energies,
+  // which only the chain gather and the energy reduction read: on engines
+  // where fork_bonded() holds they run on the side stream beside the pair
+  // passes and join before this returns (in a captured graph, a parallel
+  // branch).
   // boxes = false when the previous chain gather (mode 1) already wrote the
   // tile boxes of the current positions.
   void force_passes(bool energy, hipStream_t s, bool boxes = true) {
