@@ -70,7 +70,17 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-constexpr int kWavesPerBlock = 4;
+// Settings chosen by the autotuner (prolix_amd/tune.py) for this GPU; the
+// defaults below apply to anything it leaves unset.
+#include "prx_tuned.h"
+
+// Work items (waves) per workgroup of the pair kernels.
+#ifndef PRX_PAIR_WAVES
+#define PRX_PAIR_WAVES 4
+#endif
+constexpr int kWavesPerBlock = PRX_PAIR_WAVES;
+constexpr int kPairThreads = 64 * kWavesPerBlock;
+static_assert(kWavesPerBlock >= 1 && kWavesPerBlock <= 8, "PRX_PAIR_WAVES out of range");
 
 // LDS j accumulators are read and written through volatile pointers: the
 // compiler keeps those accesses in program order (the next iteration's read of
@@ -343,7 +353,7 @@ __device__ __forceinline__ int classify(const DeviceView& v, const SlotConst& S,
   return cls;
 }
 
-__global__ __launch_bounds__(256) void k_born_pairs(DeviceView v) {
+__global__ __launch_bounds__(kPairThreads) void k_born_pairs(DeviceView v) {
   __shared__ BornLds lds[kWavesPerBlock];
   BornLds& L = lds[threadIdx.x >> 6];
   const int w = wave_id();
@@ -529,7 +539,7 @@ __device__ __forceinline__ void force_tile(const DeviceView& v, const PairItem& 
 }
 
 template <bool ENERGY>
-__global__ __launch_bounds__(256) void k_force_pairs(DeviceView v) {
+__global__ __launch_bounds__(kPairThreads) void k_force_pairs(DeviceView v) {
   __shared__ PairLds lds[kWavesPerBlock];
   PairLds& L = lds[threadIdx.x >> 6];
   const int w = wave_id();
@@ -654,7 +664,7 @@ __device__ __forceinline__ void chain_tile(const DeviceView& v, const PairItem& 
 #endif
 }
 
-__global__ __launch_bounds__(256) void k_chain_pairs(DeviceView v) {
+__global__ __launch_bounds__(kPairThreads) void k_chain_pairs(DeviceView v) {
   __shared__ PairLds lds[kWavesPerBlock];
   PairLds& L = lds[threadIdx.x >> 6];
   const int w = wave_id();
@@ -684,24 +694,24 @@ __global__ __launch_bounds__(256) void k_chain_pairs(DeviceView v) {
   }
 }
 
-static inline int pair_blocks(int n_items) { return (n_items + 3) / 4; }
+static inline int pair_blocks(int n_items) { return (n_items + kWavesPerBlock - 1) / kWavesPerBlock; }
 
 void launch_born_pairs(const DeviceView& v, hipStream_t s) {
   if (v.n_items == 0) return;
-  hipLaunchKernelGGL(k_born_pairs, dim3(pair_blocks(v.n_items)), dim3(256), 0, s, v);
+  hipLaunchKernelGGL(k_born_pairs, dim3(pair_blocks(v.n_items)), dim3(kPairThreads), 0, s, v);
 }
 
 void launch_force_pairs(const DeviceView& v, bool energy, hipStream_t s) {
   if (v.n_items == 0) return;
   if (energy)
-    hipLaunchKernelGGL(k_force_pairs<true>, dim3(pair_blocks(v.n_items)), dim3(256), 0, s, v);
+    hipLaunchKernelGGL(k_force_pairs<true>, dim3(pair_blocks(v.n_items)), dim3(kPairThreads), 0, s, v);
   else
-    hipLaunchKernelGGL(k_force_pairs<false>, dim3(pair_blocks(v.n_items)), dim3(256), 0, s, v);
+    hipLaunchKernelGGL(k_force_pairs<false>, dim3(pair_blocks(v.n_items)), dim3(kPairThreads), 0, s, v);
 }
 
 void launch_chain_pairs(const DeviceView& v, hipStream_t s) {
   if (v.n_items == 0) return;
-  hipLaunchKernelGGL(k_chain_pairs, dim3(pair_blocks(v.n_items)), dim3(256), 0, s, v);
+  hipLaunchKernelGGL(k_chain_pairs, dim3(pair_blocks(v.n_items)), dim3(kPairThreads), 0, s, v);
 }
 
 }  // namespace prx
