@@ -772,3 +772,29 @@ def fire_minimize(force_fn, pos, mass, steps, dt_start, dt_max, force_cap=None, 
             x = x0 + dr * sc
             p = p * sc
     return x, p, f, dt, alpha, n_pos
+
+
+# ----------------------------------------------------------------------------
+# Capped steepest descent: the first stage of the batched minimiser
+# (batched_simulate.py:746-787), float64, one system
+# ----------------------------------------------------------------------------
+
+def capped_descent(force_fn, pos, steps, step_size=0.002, max_dx=0.01, k_restraint=0.0, restrained=None):
+    """Positions after ``steps`` steps of dr = F * step_size, |dr| <= max_dx per atom
+    (|dr| = sqrt(dr.dr + 1e-30)), with the optional restraint -k (r - r0) on the
+    flagged atoms; non-finite forces count as 0, and non-finite final positions
+    roll back to the start."""
+    x0 = np.asarray(pos, np.float64).copy()
+    x = x0.copy()
+    sel = None
+    if k_restraint > 0.0:
+        sel = np.ones(len(x)) if restrained is None else np.asarray(restrained, np.float64).reshape(-1)
+    for _ in range(int(steps)):
+        f = force_fn(x)
+        if sel is not None:
+            f = f - k_restraint * (x - x0) * sel[:, None]
+        f = np.where(np.isfinite(f), f, 0.0)
+        dr = f * step_size
+        mag = np.sqrt(np.sum(dr * dr, axis=1, keepdims=True) + 1e-30)
+        x = x + dr * np.minimum(1.0, max_dx / mag)
+    return x if np.isfinite(x).all() else x0

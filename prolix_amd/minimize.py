@@ -58,6 +58,41 @@ def minimize(engine, positions, steps: int = 200, dt_max: float = 4e-3, max_disp
     return x
 
 
+def capped_descent(engine, positions, steps: int = 5000, step_size: float = 0.002, max_dx: float = 0.01,
+                   k_restraint: float = 0.0, restrained=None):
+    """Steepest descent with a per-atom displacement cap, batched on the device.
+
+    Restates the first stage of prolix's batched minimiser
+    (batched_simulate.py:746-783): per step one force evaluation, then
+    dr = F * step_size with |dr| clipped to max_dx per atom (dr / |dr| computed
+    with |dr| = sqrt(dr.dr + 1e-30)), non-finite forces set to 0. An optional
+    harmonic restraint -k (r - r0) toward the starting positions acts on the
+    atoms flagged in ``restrained`` (the reference restrains heavy atoms at
+    500 kcal/mol/A^2). A system whose positions turn non-finite is returned at
+    its starting positions (the reference's rollback, batched_simulate.py:785-787).
+    Returns the packed positions (device float32 tensor).
+    """
+    t = engine.torch
+    x0 = engine._pos_tensor(positions).clone()
+    x = x0.clone()
+    sel = None
+    if k_restraint > 0.0:
+        sel = t.ones(engine.n_atoms, dtype=x.dtype, device=x.device) if restrained is None else \
+            t.as_tensor(np.asarray(restrained, np.float32).reshape(-1), device=x.device)
+    for _ in range(int(steps)):
+        _, f, _ = engine.energy_forces(x, energies=False)
+        if sel is not None:
+            f = f - float(k_restraint) * (x - x0) * sel[:, None]
+        f = t.where(t.isfinite(f), f, t.zeros_like(f))
+        dr = f * float(step_size)
+        mag = t.sqrt((dr * dr).sum(1, keepdim=True) + 1e-30)
+        x = x + dr * t.clamp(float(max_dx) / mag, max=1.0)
+    finite = t.isfinite(x).all(1).to(t.float64)
+    ok = t.segment_reduce(finite, "min", lengths=_lengths(engine)) > 0.5
+    sid = t.as_tensor(np.repeat(np.arange(engine.n_systems), engine.sizes), device=x.device)
+    return t.where(ok[sid, None], x, x0)
+
+
 # ----------------------------------------------------------------------------
 # FIRE (Bitzek et al., PRL 97, 170201 (2006)) in the form of
 # jax_md.minimize.fire_descent, which prolix's production path stages at

@@ -292,3 +292,40 @@ def test_fire_caps_bound_every_step():
         x_new = O.fire_minimize(force, x, np.ones(5), 1, 0.05, 0.5, force_cap=10.0, disp_cap=0.1)[0]
         assert np.max(np.linalg.norm(x_new - x, axis=1)) <= 0.1 + 1e-12
         x = x_new
+
+
+def test_capped_descent_bowl_and_cap():
+    """Harmonic bowl F = -kappa (x - c): every step moves an atom by
+    min(kappa h |x - c|, max_dx) straight toward c, never past it."""
+    rng = np.random.default_rng(5)
+    c = rng.normal(size=(6, 3))
+    x0 = c + rng.normal(size=(6, 3)) * 2.0
+    kappa, h, cap = 3.0, 0.002, 0.01
+    ff = lambda x: -kappa * (x - c)
+    x = x0.copy()
+    for _ in range(40):
+        x1 = O.capped_descent(ff, x, 1, step_size=h, max_dx=cap)
+        d0, d1 = np.linalg.norm(x - c, axis=1), np.linalg.norm(x1 - c, axis=1)
+        step = np.minimum(kappa * h * d0, cap)
+        assert np.allclose(d0 - d1, step, rtol=1e-9, atol=1e-12)
+        x = x1
+    assert np.allclose(O.capped_descent(ff, x0, 40, step_size=h, max_dx=cap), x, atol=1e-12)
+
+
+def test_capped_descent_restraint_balances_a_push():
+    """A constant push F0 against the restraint -k (r - r0) settles at
+    r0 + F0 / k on the restrained atoms; unrestrained atoms keep moving."""
+    x0 = np.zeros((4, 3))
+    f0 = np.array([[3.0, 0.0, 0.0], [0.0, -2.0, 0.0], [0.0, 0.0, 1.0], [1.0, 1.0, 1.0]])
+    sel = np.array([1.0, 1.0, 1.0, 0.0])
+    x = O.capped_descent(lambda x: f0, x0, 3000, step_size=0.002, max_dx=0.01, k_restraint=50.0, restrained=sel)
+    assert np.allclose(x[:3], f0[:3] / 50.0, atol=1e-9)
+    assert np.allclose(x[3], 3000 * 0.002 * f0[3], rtol=1e-12)
+
+
+def test_capped_descent_ignores_non_finite_forces():
+    x0 = np.ones((3, 3))
+    f = np.array([[np.inf, 0, 0], [np.nan, 1.0, 0], [0.0, 0.0, -1.0]])
+    x = O.capped_descent(lambda x: f, x0, 1, step_size=0.002, max_dx=0.01)
+    assert np.isfinite(x).all()
+    assert np.allclose(x - x0, [[0, 0, 0], [0, 0.002, 0], [0, 0, -0.002]])
