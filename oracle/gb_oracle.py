@@ -27,6 +27,8 @@ under prolix's ``src/prolix/``):
 - ``gb_cutoff`` options        physics/generalized_born.py:266-415 (neighbour-list
                                Born radii and GB energy, the list taken as every
                                pair closer than the cutoff)
+- ``fire_minimize``            simulate.py:615-748 (jax_md fire_descent, capped apply)
+- ``capped_descent``           batched_simulate.py:746-787 (per-atom capped SD)
 
 Forces are this module's own analytic derivatives of those energies. Tests pin
 them with central finite differences of the energies in float64, Newton's
@@ -36,7 +38,8 @@ The Born-radius chain rule is F = -(dE/dr|_B + sum_i dE/dB_i * dB_i/dr), the
 decomposition prolix documents at physics/generalized_born.py:651-706.
 
 Pinning status: every function here is checked by a closed form, finite
-differences or an invariant in tests/test_oracle.py.
+differences or an invariant in tests/test_oracle.py (the minimisers by a
+harmonic bowl's known path, cap bounds and a restraint balance).
 """
 
 from __future__ import annotations
