@@ -158,3 +158,32 @@ def test_make_energy_fn_neighbor_argument_and_gb_cutoff():
     nbrs = np.full((len(sysd["charges"]), 8), len(sysd["charges"]), np.int32)
     assert fn(sysd["positions"], neighbor=nbrs) == pytest.approx(expected, rel=1e-6)
     assert fn(sysd["positions"]) == fn(sysd["positions"], neighbor=nbrs)
+
+
+def test_hbm_budget_admission_queues_and_keeps_results():
+    """A budget that any two requests fit but not all: later arrivals wait for
+    memory, no more run together than the budget holds, and every request ends
+    in exactly the state it reaches with an unlimited budget (its trajectory
+    does not depend on what else shares the device)."""
+    from prolix_amd.serving import _bonded, hbm_bytes, poisson_trace, serve
+
+    systems = [protein(int(n), 120 + k, radius=synth.compact_radius(int(n)))
+               for k, n in enumerate((300, 700, 450, 900, 350))]
+    reqs = poisson_trace(systems, rate_per_s=1000.0, steps_range=(60, 200), seed=8)
+    costs = [hbm_bytes(len(r.system["charges"]), _bonded(r.system)) for r in reqs]
+    budget = sorted(costs)[-1] + sorted(costs)[-2] + 1.0
+    fit = max(k for k in range(1, len(costs) + 1) if sum(sorted(costs)[:k]) <= budget)
+    assert fit < len(reqs)
+    print("[gpu] serve with a binding HBM budget", flush=True)
+    tight, _ = serve(reqs, hbm_budget_bytes=budget, chunk_steps=40)
+    free, _ = serve(reqs, hbm_budget_bytes=1e15, chunk_steps=40)
+    assert any(r.admitted - r.arrival > 1e-3 for r in tight)
+    events = sorted([(r.admitted, 1) for r in tight] + [(r.finished, -1) for r in tight])
+    live = peak = 0
+    for _, d in events:
+        live += d
+        peak = max(peak, live)
+    assert peak <= fit
+    for a, b in zip(tight, free):
+        assert a.rid == b.rid and a.n_steps == b.n_steps
+        assert np.array_equal(a.positions, b.positions) and np.array_equal(a.momenta, b.momenta)
