@@ -38,9 +38,21 @@ namespace prx {
 // holds the value that started in lane (l + k) & 63.
 #define PRX_DPP_WAVE_ROL1 0x134
 
+// PRX_ROT_BPERM = 1 moves the value through the LDS crossbar
+// (ds_bpermute_b32: lane l reads lane (l + 1) & 63), which takes no VALU issue
+// slot; 0 uses the DPP wave rotate.
+#ifndef PRX_ROT_BPERM
+#define PRX_ROT_BPERM 0
+#endif
+
 __device__ __forceinline__ float rot(float x) {
+#if PRX_ROT_BPERM
+  const int src = ((int)(threadIdx.x + 1) & 63) << 2;
+  return __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(x)));
+#else
   // Every lane has a valid source under a full-wave rotate, so no "old" value is needed.
   return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), PRX_DPP_WAVE_ROL1, 0xF, 0xF, true));
+#endif
 }
 
 // Partial-buffer slot of (item w, side, lane); side 0 = i atoms, 1 = j atoms.
