@@ -106,9 +106,10 @@ def pass_timings(eng, n_steps):
         return eng.profile_passes(10), "HIP events around 10 back-to-back launches of each pass"
 
 
-def single_system(sysd, gpu_steps, cpu_steps, label, with_cpu, repeats=3):
-    """One system alone: the plain GPU path (its own engine, graph-captured
-    steps) and, beside it, the C++ restatement on the host cores."""
+def single_system(sysd, gpu_steps, label):
+    """One system alone on the plain GPU path (its own engine, graph-captured
+    steps). Returns the result and the relaxed start positions, from which the
+    CPU baseline leg times the C++ restatement on the same system."""
     import torch
 
     from prolix_amd.engine import GBEngine, Langevin
@@ -127,27 +128,29 @@ def single_system(sysd, gpu_steps, cpu_steps, label, with_cpu, repeats=3):
     eng.close()
     out = {"workload": label, "atoms": n, "gpu_steps": gpu_steps, "gpu_ms_per_step": g * 1e3,
            "gpu_ns_per_day": ns_per_day(1.0 / g), "gpu_pair_interactions_per_s": n * (n - 1) / 2 / g}
-    if with_cpu:
-        from oracle import cpu as C
+    return out, np.ascontiguousarray(x.cpu().numpy(), np.float32)
 
-        c = C.CpuSystem(sysd)
-        xc = np.ascontiguousarray(x.cpu().numpy(), np.float32)
-        _, f = c.energy_forces(xc, np.float32)
-        pc = np.zeros_like(xc)
-        c.baoab(xc, pc, f, 1, DT_FS, 300.0, 1.0, 7, dtype=np.float32)  # warm-up
 
-        def run():
-            t0 = time.perf_counter()
-            c.baoab(xc, pc, f, cpu_steps, DT_FS, 300.0, 1.0, 7, dtype=np.float32)
-            return time.perf_counter() - t0
+def cpu_single(sysd, x, cpu_steps, gpu_steps, repeats=3):
+    """The C++ restatement on one system from the same start (CPU baseline leg)."""
+    from oracle import cpu as C
 
-        tc = float(np.median([run() for _ in range(repeats)])) / cpu_steps
-        out.update({"cpu_steps": cpu_steps, "cpu_ms_per_step": tc * 1e3, "cpu_ns_per_day": ns_per_day(1.0 / tc),
-                    "cpu_cores": C.threads(), "cpu_kind": "port",
-                    "cpu_sample": f"median of {repeats} runs of {cpu_steps} float32 BAOAB steps after one warm-up"
-                                  + ("" if cpu_steps >= gpu_steps else f"; per-step time extrapolated to {gpu_steps} steps"),
-                    "gpu_over_cpu": tc / g})
-    return out
+    c = C.CpuSystem(sysd)
+    xc = x.copy()
+    _, f = c.energy_forces(xc, np.float32)
+    pc = np.zeros_like(xc)
+    c.baoab(xc, pc, f, 1, DT_FS, 300.0, 1.0, 7, dtype=np.float32)  # warm-up
+
+    def run():
+        t0 = time.perf_counter()
+        c.baoab(xc, pc, f, cpu_steps, DT_FS, 300.0, 1.0, 7, dtype=np.float32)
+        return time.perf_counter() - t0
+
+    tc = float(np.median([run() for _ in range(repeats)])) / cpu_steps
+    return {"cpu_steps": cpu_steps, "cpu_ms_per_step": tc * 1e3, "cpu_ns_per_day": ns_per_day(1.0 / tc),
+            "cpu_cores": C.threads(), "cpu_kind": "port",
+            "cpu_sample": f"median of {repeats} runs of {cpu_steps} float32 BAOAB steps after one warm-up"
+                          + ("" if cpu_steps >= gpu_steps else f"; per-step time extrapolated to {gpu_steps} steps")}
 
 
 def main():
@@ -347,13 +350,12 @@ def main():
             f"p99 {st['latency_p99_s']:.2f}s")
 
     # 8. one system alone (the 138-atom peptide, the 2,500-atom globule), GPU and CPU
-    single_out = []
+    single_out, single_x = [], []
     for label, sd, gsteps, csteps in singles:
-        t0 = time.time()
-        single_out.append(single_system(sd, gsteps, csteps, label, not a.no_cpu))
-        log(f"single system {single_out[-1]['atoms']} atoms: {single_out[-1]['gpu_ms_per_step']:.4f} ms/step GPU"
-            + (f", {single_out[-1]['cpu_ms_per_step']:.2f} ms/step CPU" if not a.no_cpu else "")
-            + f" ({time.time() - t0:.0f}s)")
+        r, xs = single_system(sd, gsteps, label)
+        single_out.append(r)
+        single_x.append(xs)
+        log(f"single system {r['atoms']} atoms: {r['gpu_ms_per_step']:.4f} ms/step GPU")
 
     # 9. CPU baseline (bounded sample)
     cpu = None
@@ -361,6 +363,11 @@ def main():
         t0 = time.time()
         cpu = cpu_baseline(systems, sizes.astype(np.float64))
         log(f"cpu baseline {cpu['value']:.3f} ns/day ({time.time() - t0:.0f}s)")
+        # the same restatement beside each one-system GPU leg, from the same start
+        for r, (label, sd, gsteps, csteps), xs in zip(single_out, singles, single_x):
+            r.update(cpu_single(sd, xs, csteps, gsteps))
+            r["gpu_over_cpu"] = r["cpu_ms_per_step"] / r["gpu_ms_per_step"]
+            log(f"single system {r['atoms']} atoms: {r['cpu_ms_per_step']:.2f} ms/step CPU")
 
     out = {
         "metric": METRIC,
