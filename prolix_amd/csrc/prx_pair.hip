@@ -91,6 +91,18 @@ __device__ __forceinline__ void wave_lds_sync() {
 #define PRX_PAIR_WAVES 4
 #endif
 constexpr int kWavesPerBlock = PRX_PAIR_WAVES;
+
+// Minimum waves per SIMD the compiler must fit the force and chain kernels'
+// registers to (0: no constraint; their unrolled loops otherwise take what
+// they need and run at lower occupancy).
+#ifndef PRX_MIN_WAVES
+#define PRX_MIN_WAVES 0
+#endif
+#if PRX_MIN_WAVES > 0
+#define PRX_OCCUPANCY __attribute__((amdgpu_waves_per_eu(PRX_MIN_WAVES)))
+#else
+#define PRX_OCCUPANCY
+#endif
 constexpr int kPairThreads = 64 * kWavesPerBlock;
 static_assert(kWavesPerBlock >= 1 && kWavesPerBlock <= 8, "PRX_PAIR_WAVES out of range");
 
@@ -551,7 +563,7 @@ __device__ __forceinline__ void force_tile(const DeviceView& v, const PairItem& 
 }
 
 template <bool ENERGY>
-__global__ __launch_bounds__(kPairThreads) void k_force_pairs(DeviceView v) {
+__global__ __launch_bounds__(kPairThreads) PRX_OCCUPANCY void k_force_pairs(DeviceView v) {
   __shared__ PairLds lds[kWavesPerBlock];
   PairLds& L = lds[threadIdx.x >> 6];
   const int w = wave_id();
@@ -676,7 +688,7 @@ __device__ __forceinline__ void chain_tile(const DeviceView& v, const PairItem& 
 #endif
 }
 
-__global__ __launch_bounds__(kPairThreads) void k_chain_pairs(DeviceView v) {
+__global__ __launch_bounds__(kPairThreads) PRX_OCCUPANCY void k_chain_pairs(DeviceView v) {
   __shared__ PairLds lds[kWavesPerBlock];
   PairLds& L = lds[threadIdx.x >> 6];
   const int w = wave_id();
