@@ -54,3 +54,14 @@ def test_pick_keeps_the_base_within_noise(tmp_path, monkeypatch, other_ms, expec
     assert chosen == (base if expect_base else other)
     assert tune.parse_tuned((tmp_path / "prx_tuned.h").read_text()) == chosen
     assert "(chosen)" in table
+
+
+def test_header_yields_to_compiler_defines():
+    """A candidate's -D settings must win over the committed choice: every
+    define in the generated (and the committed) header is guarded."""
+    text = tune.header(dict(tune.DEFAULTS))
+    for k in tune.KNOBS:
+        assert f"#ifndef {k}\n#define {k} " in text
+    committed = tune.TUNED.read_text()
+    for k in tune.parse_tuned(committed):
+        assert f"#ifndef {k}\n#define {k} " in committed, k
