@@ -65,3 +65,20 @@ def test_header_yields_to_compiler_defines():
     committed = tune.TUNED.read_text()
     for k in tune.parse_tuned(committed):
         assert f"#ifndef {k}\n#define {k} " in committed, k
+
+
+def test_build_makes_a_loadable_candidate(tmp_path, monkeypatch):
+    """One candidate compiled by the tuner loads and exports the whole ABI."""
+    import ctypes
+
+    from tests.test_host import declared_symbols
+
+    monkeypatch.setattr(tune, "TUNE_DIR", tmp_path)
+    cand = dict(tune.DEFAULTS)
+    cand["PRX_UNROLL_FORCE"] = 2
+    (d,) = tune.build([cand])
+    so = d / "libprolix_gb.so"
+    assert so.exists() and json.loads((d / "settings.json").read_text()) == cand
+    lib = ctypes.CDLL(str(so))
+    for name in declared_symbols():
+        assert hasattr(lib, name), name
