@@ -37,9 +37,24 @@ third law and rigid-motion invariance.
 The Born-radius chain rule is F = -(dE/dr|_B + sum_i dE/dB_i * dB_i/dr), the
 decomposition prolix documents at physics/generalized_born.py:651-706.
 
-Pinning status: every function here is checked by a closed form, finite
-differences or an invariant in tests/test_oracle.py (the minimisers by a
-harmonic bowl's known path, cap bounds and a restraint balance).
+Pinning status: every function here is checked in tests/test_oracle.py by a
+value fixed outside this module:
+
+- ``pair_integral``: the HCT shell quadrature of the descreening volume
+  integral, and the far-field atanh closed form;
+- ``born_radii``: the single ion (B = R - 0.09), and OBC II's published
+  alpha / beta / gamma applied to quadrature sums at psi up to 0.46;
+- ``gb_energy``: f_GB at r -> infinity, r = 2 sqrt(B_i B_j) and r = 0, with
+  the Born radii given;
+- ``ace_per_atom``: one ion by hand from the 0.0054 kcal/mol/A^2 surface
+  tension;
+- ``coulomb_energy``, ``lj_energy``, ``bond_energy``, ``angle_energy``: the
+  reference's own closed-form tests;
+- ``dihedral_angles``, ``torsion_energy``, ``harmonic_improper_energy``,
+  ``exception_energy``: constructed geometries of known dihedral / distance;
+- CMAP: node interpolation, C1 periodicity, the analytic spline of a sinusoid;
+- ``forces``: central finite differences, Newton's third law, rigid motion;
+- the minimisers: a harmonic bowl's known path, cap bounds, a restraint balance.
 """
 
 from __future__ import annotations

@@ -329,3 +329,148 @@ def test_capped_descent_ignores_non_finite_forces():
     x = O.capped_descent(lambda x: f, x0, 1, step_size=0.002, max_dx=0.01)
     assert np.isfinite(x).all()
     assert np.allclose(x - x0, [[0, 0, 0], [0, 0.002, 0], [0, 0, -0.002]])
+
+
+# ------------------------------------------- independent pins of the formulas
+#
+# Each check below derives the expected value from the underlying definition
+# or the published constants, not from the oracle's own expression, and is
+# chosen so that a plausible slip in the oracle (a dropped term, a wrong sign,
+# a wrong denominator or exponent) fails it.
+
+def _descreening_quadrature(r, rho, s):
+    """HCT descreening integral of sphere j (radius s, centre at distance r > s)
+    seen from atom i, outside i's offset sphere rho:
+    (1/4pi) * integral over sphere_j minus ball(rho) of |x|^-4 dV, done as a
+    1-D quadrature over spherical shells t around atom i. The shell of radius
+    t meets sphere j in a cap of solid-angle fraction (1 - cos theta) / 2 with
+    cos theta = (t^2 + r^2 - s^2) / (2 t r)."""
+    from scipy.integrate import quad
+
+    lo, hi = max(rho, r - s), r + s
+    if hi <= lo:
+        return 0.0
+    f = lambda t: 0.5 * (1.0 - (t * t + r * r - s * s) / (2.0 * t * r)) / (t * t)
+    val, err = quad(f, lo, hi, epsabs=0.0, epsrel=1e-13, limit=200)
+    return val
+
+
+def test_pair_integral_equals_shell_quadrature():
+    """Both branches of L (partial overlap with the offset sphere, and a sphere
+    j entirely outside it) agree with the volume integral to 1e-11."""
+    cases = [(2.5, 1.41, 1.275), (2.0, 1.61, 1.1), (3.0, 1.21, 1.5), (6.0, 1.41, 1.3), (1.9, 1.0, 0.95)]
+    for r, rho, s in cases:
+        assert r > s
+        q = _descreening_quadrature(r, rho, s)
+        assert O.pair_integral(np.array(r), rho, s) == pytest.approx(q, rel=1e-11), (r, rho, s)
+
+
+def test_obc2_radii_at_positive_psi_from_quadrature():
+    """OBC II rescale (Onufriev, Bashford & Case, Proteins 55:383, 2004: alpha = 1,
+    beta = 0.8, gamma = 4.85) on descreening sums taken by quadrature. A buried
+    atom inside an icosahedral shell reaches psi = 0.46 (shell atoms 0.2-0.3,
+    outer atoms 0.04-0.08), where tanh is far from its argument, so the three
+    coefficients, the sign of beta and the 1/R (not 1/rho) denominator all
+    decide B_i. Every pair has r > s_j, where prolix's closed form is the
+    volume integral."""
+    g = (1 + 5**0.5) / 2
+    ico = np.array([[0, 1, g], [0, -1, g], [0, 1, -g], [0, -1, -g], [1, g, 0], [-1, g, 0], [1, -g, 0],
+                    [-1, -g, 0], [g, 0, 1], [-g, 0, 1], [g, 0, -1], [-g, 0, -1]], float)
+    ico /= np.linalg.norm(ico, axis=1, keepdims=True)
+    pos = np.vstack([[0.0, 0, 0], ico * 2.3, ico[[0, 3, 5, 8]] * 4.3 + 0.3])
+    n = len(pos)
+    rng = np.random.default_rng(3)
+    R = rng.uniform(1.3, 1.9, n)
+    s = R * rng.uniform(0.72, 0.85, n)
+    rho = R - 0.09
+    I = np.zeros(n)
+    for i in range(n):
+        for j in range(n):
+            if i != j:
+                r = float(np.linalg.norm(pos[i] - pos[j]))
+                assert r > s[j]
+                I[i] += _descreening_quadrature(r, rho[i], s[j])
+    psi = rho * I
+    assert psi.max() > 0.4 and psi.min() > 0.03
+    alpha, beta, gamma = 1.0, 0.8, 4.85
+    expect = 1.0 / (1.0 / rho - np.tanh(alpha * psi - beta * psi**2 + gamma * psi**3) / R)
+    got = O.born_radii(pos, R, s)
+    assert np.allclose(got, expect, rtol=1e-10, atol=0)
+    # a flipped beta or a 1/rho denominator moves the buried atom's B by > 1%
+    wrong_beta = 1.0 / (1.0 / rho - np.tanh(psi + beta * psi**2 + gamma * psi**3) / R)
+    wrong_den = 1.0 / (1.0 / rho - np.tanh(psi - beta * psi**2 + gamma * psi**3) / rho)
+    assert abs(wrong_beta[0] / got[0] - 1) > 1e-2 and abs(wrong_den[0] / got[0] - 1) > 1e-2
+
+
+def test_gb_pair_function_limits():
+    """f_GB = sqrt(r^2 + B_i B_j exp(-r^2 / (4 B_i B_j))) (Still et al. 1990) at
+    three points fixed by hand, with the Born radii given: widely separated ions
+    (f -> r, each unordered pair counted in both orders), r = 2 sqrt(B_i B_j)
+    (the exponent is exactly -1) and coincident atoms (f = sqrt(B_i B_j))."""
+    q = np.array([0.7, -0.45])
+    B = np.array([1.4, 1.9])
+    pref = -0.5 * 332.0637 * (1.0 - 1.0 / 78.5)
+    self_term = q[0] ** 2 / B[0] + q[1] ** 2 / B[1]
+    bb = B[0] * B[1]
+    for r, f01 in ((400.0, 400.0), (2.0 * math.sqrt(bb), math.sqrt(bb * (4.0 + math.exp(-1.0)))),
+                   (0.0, math.sqrt(bb))):
+        pos = np.array([[0.0, 0, 0], [r, 0, 0]])
+        e = O.gb_energy(pos, q, B, eps_in=1.0, eps_out=78.5)
+        assert e == pytest.approx(pref * (self_term + 2.0 * q[0] * q[1] / f01), rel=1e-11), r
+
+
+def test_ace_single_ion_by_hand():
+    """ACE for one ion of radius 1.5 A (OpenMM's ACE form): B = R - 0.09 = 1.41 A
+    for an isolated atom; offset radius in nm 0.141 + 0.009 = 0.15; probe 0.14 nm;
+    coefficient 4 pi x 2.25936 kJ/mol/nm^2 (surface tension 0.0054 kcal/mol/A^2);
+    kJ -> kcal with prolix's 0.239006."""
+    s = {"charges": np.array([0.0]), "sigmas": np.ones(1), "epsilons": np.zeros(1), "radii": np.array([1.5])}
+    t = O.energy_terms(s, np.zeros((1, 3)))
+    coeff = 4.0 * math.pi * (0.0054 * 4.184 * 100.0)
+    expect = coeff * (0.15 + 0.14) ** 2 * (0.15 / 0.141) ** 6 * 0.239006
+    assert t["ace"] == pytest.approx(expect, rel=2e-8)
+    assert t["gb"] == 0.0  # uncharged
+
+
+def _dihedral_geometry(phi):
+    """Atoms i, j, k, l with j at the origin, k on +z and i on +x; l is i's
+    direction turned by phi about the j -> k axis, so the IUPAC dihedral is phi."""
+    return np.array([[1.0, 0, 0], [0.0, 0, 0], [0.0, 0, 1.0], [math.cos(phi), math.sin(phi), 1.0]])
+
+
+def test_dihedral_angle_sign_and_value():
+    idx = np.array([[0, 1, 2, 3]])
+    for phi in (math.pi / 3, -2.0, 2.9, -0.4):
+        assert O.dihedral_angles(_dihedral_geometry(phi), idx)[0] == pytest.approx(phi, abs=1e-7)
+
+
+def test_periodic_torsion_by_hand():
+    """k (1 + cos(n phi - delta)) at phi = pi / 3 (bonded.py:110-141): n = 1,
+    delta = pi/3 -> 2k; n = 3, delta = 0 -> 0; n = 2, delta = pi -> k (1 + cos(-pi/3))
+    = 1.5 k. Terms of one torsion add."""
+    pos = _dihedral_geometry(math.pi / 3)
+    idx = np.array([[0, 1, 2, 3]])
+    prm = np.array([[[1.0, math.pi / 3, 2.0], [3.0, 0.0, 5.0], [2.0, math.pi, 0.8]]])
+    assert O.torsion_energy(pos, idx, prm) == pytest.approx(4.0 + 0.0 + 1.2, abs=1e-6)
+
+
+def test_harmonic_improper_by_hand():
+    """k wrap(phi - phi0)^2 with no 1/2 (bonded.py:168-183), wrapped to [-pi, pi)."""
+    idx = np.array([[0, 1, 2, 3]])
+    cases = [(math.pi / 3, -math.pi / 2, (5 * math.pi / 6) ** 2),
+             (math.pi / 3, 5 * math.pi / 6, (math.pi / 2) ** 2),
+             (math.radians(170), math.radians(-170), math.radians(20) ** 2)]
+    for phi, phi0, d2 in cases:
+        e = O.harmonic_improper_energy(_dihedral_geometry(phi), idx, np.array([[[7.0, phi0]]]))
+        assert e == pytest.approx(7.0 * d2, rel=1e-6), (phi, phi0)
+
+
+def test_exception_pair_by_hand():
+    """Pre-scaled 1-4 pair (bonded.py:188-228): LJ is 0 at r = sigma and -eps at
+    the minimum r = 2^(1/6) sigma; Coulomb k qq / r."""
+    sig, eps, qq = 3.1, 0.15, -0.12
+    pairs = np.array([[0, 1]])
+    for r, lj in ((sig, 0.0), (2.0 ** (1.0 / 6.0) * sig, -eps)):
+        pos = np.array([[0.0, 0, 0], [0.0, r, 0]])
+        e = O.exception_energy(pos, pairs, np.array([sig]), np.array([eps]), np.array([qq]))
+        assert e == pytest.approx(lj + 332.0637 * qq / r, rel=1e-9, abs=1e-12)
