@@ -188,6 +188,43 @@ int prx_recenter_com(prx_engine* eng, void* stream);
  * force, constraint and displacement caps; warn_counts of typing.py:558-563).
  * Host out, 4 values; synchronises the stream. */
 int prx_integrator_warnings(prx_engine* eng, int32_t system, int64_t* counts);
+/* Noise-stream position of each system (host array, one entry per system in
+ * packed order, each in [0, 2^32)): the number of BAOAB steps it has taken.
+ * Called after prx_integrator_init with the momenta of a saved state, it
+ * resumes that state's Langevin trajectory bit for bit (the step counter is
+ * the Philox counter), as the resumed production of
+ * batched_simulate.py:1300-1305 continues from a saved state and step. */
+int prx_integrator_set_step_counts(prx_engine* eng, const int64_t* steps, void* stream);
+/* Explosion check (simulate.py:1050-1071, batched_simulate.py:158-205) kept
+ * on the device: nonfinite receives, per system in packed order (host array),
+ * the number of atom-steps since its prx_integrator_init / admission that
+ * ended with a non-finite position or momentum. Synchronises the stream. */
+int prx_integrator_health(prx_engine* eng, int64_t* nonfinite, void* stream);
+/* Verlet skin (A, default 0) of the neighbour lists of the engine's gb_cut
+ * systems (cutoff > 0 with gb_cutoff = 1), which take their pair work from a
+ * list built on the device: per 64-atom tile, the atoms of later tiles
+ * within cutoff + skin of its bounding box. With skin = 0 the list is rebuilt
+ * at every evaluation; with skin > 0 when an atom has moved more than skin / 2
+ * since the last build (and at every eager evaluation and graph launch).
+ * Results do not depend on the skin: every pair is tested against the cutoff.
+ * Replaces make_neighbor_list_fn's jax_md list (physics/neighbor_list.py:155-172,
+ * dr_threshold) and its rebuild rule. */
+int prx_engine_set_nl_skin(prx_engine* eng, double skin);
+/* How prx_integrator_steps runs the steps: 1 = captured HIP graphs of the
+ * per-pass kernels; 2 = one persistent cooperative kernel per 1,000 steps
+ * whose passes are separated by grid-wide barriers (the same per-item and
+ * per-tile code, so the trajectory is the same bit for bit; it removes the
+ * per-pass launch latency that bounds small engines, and the call returns
+ * after the steps are done); 0 = auto (default): persistent for engines of
+ * at most 16,384 work items, graphs above. PRX_ERR_STATE for 2 where
+ * cooperative launches are unavailable. No counterpart in prolix (its loop
+ * is jax.lax.fori_loop, physics/simulate.py:253-254). */
+int prx_engine_set_step_mode(prx_engine* eng, int32_t mode);
+/* Run the bonded kernel on a second stream beside the pair passes (a
+ * parallel branch of the step graph) when the engine holds at least
+ * min_atoms atoms; below it, in line. Default 100,000. Results are the same
+ * either way, bit for bit. No counterpart in prolix. */
+int prx_engine_set_fork_min_atoms(prx_engine* eng, int64_t min_atoms);
 
 /* ---- measurement --------------------------------------------------------
  * Launch each force pass of one step n_reps times back to back, the run
@@ -246,8 +283,10 @@ int prx_pool_energy(prx_engine* pool, int32_t slot, double* energies_host, void*
  * stored [psi, phi]: out is (G, G, 16) float64 indexed [phi, psi]. */
 int prx_cmap_coefficients(const double* grid, int32_t G, double* out);
 
-/* Number of 64x64 tile pairs (work items) and of exclusion tiles the pair
- * kernels would use for one system. */
+/* Number of work items and of exclusion tiles the pair kernels would use for
+ * one system in an engine of its own: the 64x64 tile pairs I <= J, or for a
+ * gb_cut system its diagonal and exclusion tile pairs plus the neighbour
+ * list's chunk slots. */
 int prx_plan_tiles(const prx_system_desc* desc, int64_t* n_items, int64_t* n_excl_tiles);
 
 #ifdef __cplusplus

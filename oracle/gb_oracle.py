@@ -29,6 +29,7 @@ under prolix's ``src/prolix/``):
                                pair closer than the cutoff)
 - ``fire_minimize``            simulate.py:615-748 (jax_md fire_descent, capped apply)
 - ``capped_descent``           batched_simulate.py:746-787 (per-atom capped SD)
+- ``sd_backtrack_step``        physics/simulate.py:141-196 (SD + 4-point backtracking)
 
 Forces are this module's own analytic derivatives of those energies. Tests pin
 them with central finite differences of the energies in float64, Newton's
@@ -54,7 +55,9 @@ value fixed outside this module:
   ``exception_energy``: constructed geometries of known dihedral / distance;
 - CMAP: node interpolation, C1 periodicity, the analytic spline of a sinusoid;
 - ``forces``: central finite differences, Newton's third law, rigid motion;
-- the minimisers: a harmonic bowl's known path, cap bounds, a restraint balance.
+- the minimisers: a harmonic bowl's known path, cap bounds, a restraint balance;
+  the backtracking search: a bowl whose candidate energies are known in
+  closed form, so the selection order and the strict comparisons are fixed.
 """
 
 from __future__ import annotations
@@ -816,3 +819,33 @@ def capped_descent(force_fn, pos, steps, step_size=0.002, max_dx=0.01, k_restrai
         mag = np.sqrt(np.sum(dr * dr, axis=1, keepdims=True) + 1e-30)
         x = x + dr * np.minimum(1.0, max_dx / mag)
     return x if np.isfinite(x).all() else x0
+
+
+# ----------------------------------------------------------------------------
+# Steepest descent with a 4-point backtracking search (physics/simulate.py:141-196),
+# float64, one system
+# ----------------------------------------------------------------------------
+
+SD_FRACTIONS = (1.0, 0.5, 0.25, 0.1)
+
+
+def sd_backtrack_step(energy_fn, force_fn, pos, dt_max=4e-3, max_displacement=0.1):
+    """One iteration of physics/simulate.py:174-191: dt = min(max_disp / max|F|, dt_max)
+    with max|F| + 1e-8, candidates r + {1, 0.5, 0.25, 0.1} dt F; the 0.1
+    candidate is the start and 0.25, 0.5, 1 replace it when strictly lower; the
+    best replaces r when strictly below E(r). Returns (positions, chosen fraction
+    or None, the four candidate energies, E(r))."""
+    x = np.asarray(pos, np.float64)
+    e0 = float(energy_fn(x))
+    f = force_fn(x)
+    fmax = float(np.max(np.linalg.norm(f, axis=1))) + 1e-8
+    dt = min(max_displacement / fmax, dt_max)
+    cands = [x + frac * dt * f for frac in SD_FRACTIONS]
+    es = [float(energy_fn(c)) for c in cands]
+    best = 3
+    for k in (2, 1, 0):
+        if es[k] < es[best]:
+            best = k
+    if es[best] < e0:
+        return cands[best], SD_FRACTIONS[best], es, e0
+    return x, None, es, e0

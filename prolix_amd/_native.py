@@ -80,6 +80,12 @@ _SIGS = {
     "prx_recenter_com": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "prx_integrator_step_count": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64)]),
     "prx_integrator_warnings": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64)]),
+    "prx_integrator_set_step_counts": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64),
+                                                      ctypes.c_void_p]),
+    "prx_integrator_health": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64), ctypes.c_void_p]),
+    "prx_engine_set_fork_min_atoms": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64]),
+    "prx_engine_set_nl_skin": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
+    "prx_engine_set_step_mode": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "prx_pool_create": (ctypes.c_int, [ctypes.POINTER(PoolLimits), ctypes.POINTER(ctypes.c_void_p)]),
     "prx_pool_admit": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SystemDesc), c_f32p,
                                       ctypes.POINTER(LangevinParams), c_i32p, ctypes.c_void_p]),

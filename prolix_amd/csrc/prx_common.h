@@ -70,6 +70,11 @@ struct SlotConst {
   int32_t caps;         // 1: batched-step safety caps (velocity, displacement, force)
   int32_t tile0;        // global index of the system's first tile
   uint64_t seed;
+  // neighbour-list systems (gb_cut with a cutoff): list build and reuse
+  int32_t nl;           // 1: pair work from the list (chunk items), 0: all tile pairs
+  int32_t tp_base;      // global index of the system's first tile pair (I < J) record
+  float nl_cut2;        // (cutoff + skin)^2 with a 1e-5 relative margin: list membership
+  float skin_half2;     // (skin / 2)^2; 0: the list is rebuilt at every evaluation
 };
 
 // Safety caps of prolix's batched BAOAB step (batched_simulate.py:269-358), in
@@ -79,6 +84,11 @@ constexpr float kVLimit = 20.0f;
 constexpr float kMaxDx = 0.5f;
 constexpr float kForceCap = 10000.0f;
 enum { kWarnVLimit = 0, kWarnForceCap = 1, kWarnConstr = 2, kWarnDxCap = 3 };
+// Per-slot health counters that are not safety caps: atom-steps that ended
+// with a non-finite position or momentum (the explosion check of
+// batched_simulate.py:158-205 and simulate.py:1050-1071, kept on the device).
+enum { kWarnNonFinite = 4 };
+constexpr int kWarnStride = 8;  // counters per slot
 
 // Bonded entries; idx.x < 0 marks a free arena slot.
 struct BondEntry { int4 idx; float4 p; };    // idx.xy atoms, idx.z kind (0 bond, 1 UB); p.xy = r0, k
@@ -88,6 +98,22 @@ struct AngleEntry { int4 idx; float4 p; };   // idx.xyz atoms; p.xy = theta0, k
 struct TorsionEntry { int4 idx; float4 a; float4 b; };
 struct CmapEntry { int4 idx; int4 idx2; };   // idx = atoms a..d, idx2.x = atom e, .y = coeff base, .z = G
 struct ExceptionEntry { int4 idx; float4 p; };  // idx.xy pair; p = (sigma, eps, qq * k_coulomb, 0)
+
+// Neighbour list of a gb_cut system (one with a cutoff whose Born sums and
+// GB terms are cut too). Built on the device from the tile boxes:
+//  - per tile pair (I < J) without exclusions, a 64-bit mask of the atoms of J
+//    within cutoff + skin of tile I's box (nl_mask), and the number of listed
+//    atoms of the tiles I + 1 .. J - 1 before them (nl_pref);
+//  - per tile I, its list: the masked atoms of every J > I in increasing
+//    order (nl_entries from the row's entry base, nl_count[I] of them),
+//    evaluated 64 at a time by chunk items (PairItem.j0 < 0, see below).
+// Tile pairs with exclusions and the diagonal tiles stay ordinary items.
+// Chunk item: j0 = -1 - (global entry index of its first j atom); excl = the
+// global tile index of its row; ni = atoms of the i tile.
+__host__ __device__ inline bool item_is_chunk(const PairItem& it) { return it.j0 < 0; }
+// Per-tile-pair record: I | J << 16 (local tiles), and the slot (~slot for a
+// pair with exclusions, which the list skips; x = -1 for a free record).
+struct TilePairInfo { int32_t ij; int32_t slot; };
 
 // Atoms per bonded entry kind; each entry writes one float4 force record per atom.
 constexpr int kRecBond = 2, kRecAngle = 3, kRecTors = 4, kRecCmap = 5, kRecExc = 2;
@@ -133,12 +159,36 @@ struct DeviceView {
   // systems
   SlotConst* slots;
   uint32_t* atom_step;     // per atom: BAOAB steps taken (the noise counter)
-  unsigned int* warn;      // per slot: 4 safety-cap event counters (atom-steps)
+  unsigned int* warn;      // per slot: kWarnStride counters (safety caps, non-finite atom-steps)
   int32_t max_slots;
+  // neighbour lists (gb_cut systems)
+  const TilePairInfo* nl_tp;     // per tile-pair record (static)
+  int32_t nl_tp_hw;              // tile-pair arena high-water mark
+  unsigned long long* nl_mask;   // per tile pair: listed atoms of J
+  int32_t* nl_pref;              // per tile pair: listed atoms of the row before J
+  int32_t* nl_entries;           // list entries (atom arena indices)
+  const int4* nl_row;            // per tile: (first chunk item, entry base, first, end regular row item)
+  int32_t* nl_count;             // per tile: entries in its list
+  const int2* nl_col;            // per tile: (offset, count) of its regular column items in nl_colitems
+  const int32_t* nl_colitems;    // item indices
+  float4* nl_xref;               // per atom: position at the last list build
+  int32_t* nl_flag;              // per slot: 1 = build the list at this evaluation
+};
+
+// Grid-wide barrier state of the persistent step kernel (device memory).
+struct GridBarrier {
+  unsigned count;       // blocks arrived at the current barrier
+  unsigned generation;  // barriers completed in this launch
+  int error;            // 1: a wait timed out; every block leaves the launch
+  int pad;
 };
 
 // Launchers (defined in the .hip translation units).
 void launch_tile_boxes(const DeviceView& v, hipStream_t s);                // -> tbox (every evaluation)
+// Neighbour-list build of the gb_cut systems (after the tile boxes): with
+// force, every list is rebuilt; otherwise those whose atoms moved more than
+// skin / 2 since their last build (every list, when skin = 0).
+void launch_nl_build(const DeviceView& v, const int32_t* slot_ranges, int n_slots, bool force, hipStream_t s);
 void launch_born_pairs(const DeviceView& v, hipStream_t s);                // also -> item_cls
 void launch_born_gather(const DeviceView& v, hipStream_t s);               // -> B, dB/dI
 void launch_force_pairs(const DeviceView& v, bool energy, hipStream_t s);
@@ -157,11 +207,18 @@ void launch_copy_out(const DeviceView& v, float* pos, float* mom, float* frc, co
                      hipStream_t s);
 void launch_copy_in_momenta(const DeviceView& v, const float* packed, const int32_t* map, int n,
                             hipStream_t s);
+void launch_fill_u32(uint32_t* dst, uint32_t value, int n, hipStream_t s);  // dst[0..n) = value
 void launch_init_momenta(const DeviceView& v, const int32_t* slot_list, int n_slot_list, hipStream_t s);
 void launch_kinetic(const DeviceView& v, const int32_t* slot_ranges, int n_slots, double* out,
                     hipStream_t s);
 void launch_recenter_com(const DeviceView& v, const int32_t* slot_ranges, int n_slots, hipStream_t s);
 void launch_tick(unsigned long long* out, hipStream_t s);
+// Persistent step kernel (prx_persist.hip): the largest co-resident grid of
+// its blocks (0 if cooperative launches are unavailable), and one launch of
+// n_steps steps with n_blocks blocks (<= that grid).
+int persist_max_blocks();
+hipError_t launch_persist_steps(const DeviceView& v, const int32_t* slot_ranges, int n_slots, int n_steps,
+                                bool has_nl, GridBarrier* bar, int n_blocks, hipStream_t s);
 void launch_born_out(const DeviceView& v, float* out, const int32_t* map, int n, hipStream_t s);
 
 }  // namespace prx

@@ -36,7 +36,8 @@ struct StateError : std::runtime_error {
 };
 
 // Bonded terms on a side stream beside the pair passes (1) or in line (0),
-// from this many live atoms on: below it the fork's extra graph nodes cost
+// from this many live atoms on (the default of the engine's runtime setting,
+// prx_engine_set_fork_min_atoms): below it the fork's extra graph nodes cost
 // more than the bonded kernel they hide (measured: 8% slower steps for one
 // 138- or 2,500-atom system, 1.2% faster for the 425,157-atom mix).
 #ifndef PRX_FORK_BONDED
@@ -110,7 +111,11 @@ struct Slot {
   int32_t a0 = 0, n = 0;
   int64_t bond_b = 0, bond_n = 0, ang_b = 0, ang_n = 0, tors_b = 0, tors_n = 0, cmap_b = 0, cmap_n = 0;
   int64_t exc_b = 0, exc_n = 0, coeff_b = 0, coeff_n = 0, xt_b = 0, xt_n = 0, rl_b = 0, rl_n = 0;
-  std::vector<prx::PairItem> items;  // global indices, fixed for the slot's lifetime
+  int64_t ent_b = 0, ent_n = 0, tp_b = 0, tp_n = 0;  // neighbour-list entries and tile-pair records
+  bool nl = false;
+  std::vector<int4> nl_rows;                 // per tile, relative (NlPlan::rows)
+  std::vector<std::vector<int32_t>> nl_cols;  // per tile, relative item indices
+  std::vector<prx::PairItem> items;  // global atom indices, fixed for the slot's lifetime
   prx::SlotConst c{};
   bool has_params = false;
 };
@@ -180,6 +185,21 @@ void check_idx(const int32_t* idx, int64_t rows, int cols, int n, const char* wh
       throw std::invalid_argument(std::string(what) + " index out of range");
 }
 
+// A gb_cut system (all its pair terms cut) takes its pair work from a
+// neighbour list built on the device.
+bool uses_nl(const prx_system_desc& d) { return d.gb_cutoff && d.cutoff > 0; }
+
+// Work items of one system alone, and its neighbour-list tile pairs and
+// entries (0 for a system without a list).
+int64_t plan_items(const prx_system_desc& d, const prx::ExclPlan& plan, int64_t* n_tp, int64_t* n_ent) {
+  *n_tp = *n_ent = 0;
+  const int64_t nT = (d.n_atoms + prx::kTile - 1) / prx::kTile;
+  if (!uses_nl(d)) return nT * (nT + 1) / 2;
+  int64_t items = 0;
+  prx::nl_sizes(d.n_atoms, plan, n_tp, n_ent, &items);
+  return items;
+}
+
 void validate(const prx_system_desc& d) {
   if (d.n_atoms <= 0) throw std::invalid_argument("n_atoms must be positive (charges is empty)");
   if (!d.charges || !d.sigmas || !d.epsilons || !d.radii)
@@ -218,10 +238,30 @@ void validate(const prx_system_desc& d) {
 struct prx_engine {
   // capacities
   int64_t cap_atoms = 0, cap_bond = 0, cap_ang = 0, cap_tors = 0, cap_cmap = 0, cap_exc = 0;
-  int64_t cap_coeff = 0, cap_xt = 0, cap_rl = 0;
+  int64_t cap_coeff = 0, cap_xt = 0, cap_rl = 0, cap_ent = 0, cap_tp = 0;
   int32_t cap_slots = 0;
   bool pool = false;
-  RangeAlloc a_atoms, a_bond, a_ang, a_tors, a_cmap, a_exc, a_coeff, a_xt, a_rl;
+  RangeAlloc a_atoms, a_bond, a_ang, a_tors, a_cmap, a_exc, a_coeff, a_xt, a_rl, a_ent, a_tp;
+  double nl_skin = 0.0;           // neighbour-list skin (A) of systems admitted from now on
+  // neighbour-list device arrays
+  prx::TilePairInfo* d_nl_tp = nullptr;
+  unsigned long long* d_nl_mask = nullptr;
+  int32_t* d_nl_pref = nullptr;
+  int32_t* d_nl_entries = nullptr;
+  float4* d_nl_xref = nullptr;
+  int32_t* d_nl_flag = nullptr;
+  int4* d_nl_row = nullptr;       // per tile (sized with the tiles)
+  int32_t* d_nl_count = nullptr;  // per tile
+  int2* d_nl_col = nullptr;       // per tile
+  int32_t* d_nl_colitems = nullptr;
+  int64_t cap_colitems = 0;
+  bool has_nl = false;            // a live system uses its neighbour list
+  // step path: 0 = auto (the persistent kernel for engines of at most
+  // kPersistAutoItems work items, graphs above), 1 = captured graphs,
+  // 2 = the persistent kernel
+  int32_t step_mode = 1;
+  prx::GridBarrier* d_bar = nullptr;
+  int persist_blocks = -1;        // co-resident grid of the persistent kernel (0: unavailable)
   int32_t* d_rec_list = nullptr;
   std::vector<Slot> slots;
   prx::DeviceView v{};
@@ -229,6 +269,11 @@ struct prx_engine {
   prx::PairItem* d_items = nullptr;
   int64_t cap_items = 0;
   float4* d_part = nullptr;   // 2 * 64 partials per item
+  // positions and forces of prx_energy_forces: an evaluation at arbitrary
+  // positions leaves the integrator's state (v.pos, v.force) untouched
+  float4* d_eval_pos = nullptr;
+  float4* d_eval_force = nullptr;
+  int64_t fork_min_atoms = PRX_FORK_MIN_ATOMS;
   int4* d_tiles = nullptr;
   int64_t cap_tiles = 0;
   float4* d_tbox = nullptr;
@@ -250,54 +295,114 @@ struct prx_engine {
   bool initialized = false;
   std::vector<std::pair<int, hipGraphExec_t>> graphs;  // (steps per launch, executable)
 
-  void alloc_device() {
-    v.pos = dalloc<float4>(cap_atoms);
-    v.mom = dalloc<float4>(cap_atoms);
-    v.force = dalloc<float4>(cap_atoms);
-    v.born4 = dalloc<float4>(cap_atoms);
-    v.nb4 = dalloc<float4>(cap_atoms);
-    v.fin4 = dalloc<double4>(cap_atoms);
-    v.sqrt_m = dalloc<float>(cap_atoms);
-    v.slot_of = dalloc<int32_t>(cap_atoms);
-    HIPCHECK(hipMemset(v.slot_of, 0xFF, std::max<int64_t>(cap_atoms, 1) * sizeof(int32_t)));
-    v.gb4 = dalloc<float4>(cap_atoms);
-    v.fpair = dalloc<float4>(cap_atoms);
-    v.e_atom = dalloc<double>(cap_atoms * 2);
-    v.atom_step = dalloc<uint32_t>(cap_atoms);
-    v.bonds = dalloc<prx::BondEntry>(cap_bond);
-    v.angles = dalloc<prx::AngleEntry>(cap_ang);
-    v.tors = dalloc<prx::TorsionEntry>(cap_tors);
-    v.cmaps = dalloc<prx::CmapEntry>(cap_cmap);
-    v.excs = dalloc<prx::ExceptionEntry>(cap_exc);
-    mark_free_entries();
-    v.e_bond = dalloc<double>(cap_bond);
-    v.e_angle = dalloc<double>(cap_ang);
-    v.e_tors = dalloc<double>(cap_tors);
-    v.e_cmap = dalloc<double>(cap_cmap);
-    v.e_exc = dalloc<double>(cap_exc);
+  // Arena mode (arena != nullptr): every device buffer is carved, 256-byte
+  // aligned and in a fixed order, from one caller-provided region of
+  // arena_cap bytes; the engine then never allocates or frees device memory
+  // and creates no streams (it runs on the caller's stream only).
+  char* arena = nullptr;
+  size_t arena_cap = 0, arena_used = 0;
+
+  template <class T>
+  T* get(size_t n) {
+    if (!arena) return dalloc<T>(n);
+    if (n == 0) n = 1;
+    const size_t off = (arena_used + 255) & ~(size_t)255;
+    const size_t bytes = n * sizeof(T);
+    if (off + bytes > arena_cap || off + bytes < off)
+      throw CapacityError("the system's buffers exceed the " + std::to_string(arena_cap >> 20) +
+                          " MiB scratch region");
+    arena_used = off + bytes;
+    return reinterpret_cast<T*>(arena + off);
+  }
+
+  // In arena mode the work list's buffers are sized now, exactly (the engine
+  // holds one fixed set of systems); otherwise rebuild() sizes them.
+  void alloc_work(int64_t n_items, int64_t n_tiles) {
+    cap_items = n_items;
+    d_items = get<prx::PairItem>(cap_items);
+    d_e_items = get<double>(cap_items * 3);
+    d_part = get<float4>(cap_items * 2 * prx::kTile);
+    d_item_cls = get<int32_t>(cap_items);
+    cap_tiles = n_tiles;
+    d_tiles = get<int4>(cap_tiles);
+    d_tbox = get<float4>(cap_tiles * 2);
+    d_nl_row = get<int4>(cap_tiles);
+    d_nl_count = get<int32_t>(cap_tiles);
+    d_nl_col = get<int2>(cap_tiles);
+    cap_colitems = n_items;
+    d_nl_colitems = get<int32_t>(cap_colitems);
+  }
+
+  void alloc_device(int64_t arena_items = 0, int64_t arena_tiles = 0) {
+    v.pos = get<float4>(cap_atoms);
+    v.mom = get<float4>(cap_atoms);
+    v.force = get<float4>(cap_atoms);
+    v.born4 = get<float4>(cap_atoms);
+    v.nb4 = get<float4>(cap_atoms);
+    v.fin4 = get<double4>(cap_atoms);
+    v.sqrt_m = get<float>(cap_atoms);
+    v.slot_of = get<int32_t>(cap_atoms);
+    v.gb4 = get<float4>(cap_atoms);
+    v.fpair = get<float4>(cap_atoms);
+    v.e_atom = get<double>(cap_atoms * 2);
+    v.atom_step = get<uint32_t>(cap_atoms);
+    v.bonds = get<prx::BondEntry>(cap_bond);
+    v.angles = get<prx::AngleEntry>(cap_ang);
+    v.tors = get<prx::TorsionEntry>(cap_tors);
+    v.cmaps = get<prx::CmapEntry>(cap_cmap);
+    v.excs = get<prx::ExceptionEntry>(cap_exc);
+    v.e_bond = get<double>(cap_bond);
+    v.e_angle = get<double>(cap_ang);
+    v.e_tors = get<double>(cap_tors);
+    v.e_cmap = get<double>(cap_cmap);
+    v.e_exc = get<double>(cap_exc);
     v.rec_base[0] = 0;
     v.rec_base[1] = v.rec_base[0] + (int64_t)prx::kRecBond * cap_bond;
     v.rec_base[2] = v.rec_base[1] + (int64_t)prx::kRecAngle * cap_ang;
     v.rec_base[3] = v.rec_base[2] + (int64_t)prx::kRecTors * cap_tors;
     v.rec_base[4] = v.rec_base[3] + (int64_t)prx::kRecCmap * cap_cmap;
     const int64_t n_rec = v.rec_base[4] + (int64_t)prx::kRecExc * cap_exc;
-    v.rec = dalloc<float4>(n_rec);
-    v.atom_rec = dalloc<int2>(cap_atoms);
+    v.rec = get<float4>(n_rec);
+    v.atom_rec = get<int2>(cap_atoms);
     cap_rl = std::max<int64_t>(cap_rl, 1);
-    d_rec_list = dalloc<int32_t>(cap_rl);
+    d_rec_list = get<int32_t>(cap_rl);
     v.rec_list = d_rec_list;
-    v.warn = dalloc<unsigned int>((size_t)cap_slots * 4);
-    d_coeff = dalloc<float>(cap_coeff);
+    v.warn = get<unsigned int>((size_t)cap_slots * prx::kWarnStride);
+    d_eval_pos = get<float4>(cap_atoms);
+    d_eval_force = get<float4>(cap_atoms);
+    d_coeff = get<float>(cap_coeff);
     v.cmap_coeff = d_coeff;
-    d_excl = dalloc<unsigned long long>(cap_xt * 128);
+    d_excl = get<unsigned long long>(cap_xt * 128);
     v.excl_masks = d_excl;
-    v.slots = dalloc<prx::SlotConst>(cap_slots);
+    v.slots = get<prx::SlotConst>(cap_slots);
     v.max_slots = cap_slots;
-    d_ranges = dalloc<int32_t>((size_t)cap_slots * 14);
-    d_slot_list = dalloc<int32_t>(cap_slots);
-    d_energy_tmp = dalloc<double>((size_t)cap_slots * PRX_N_ENERGY);
+    d_ranges = get<int32_t>((size_t)cap_slots * 14);
+    d_slot_list = get<int32_t>(cap_slots);
+    d_energy_tmp = get<double>((size_t)cap_slots * PRX_N_ENERGY);
     cap_map = cap_atoms;
-    d_map = dalloc<int32_t>(cap_map);
+    d_map = get<int32_t>(cap_map);
+    d_nl_tp = get<prx::TilePairInfo>(cap_tp);
+    d_nl_mask = get<unsigned long long>(cap_tp);
+    d_nl_pref = get<int32_t>(cap_tp);
+    d_nl_entries = get<int32_t>(cap_ent);
+    d_nl_xref = get<float4>(cap_atoms);
+    d_nl_flag = get<int32_t>(cap_slots);
+    d_bar = get<prx::GridBarrier>(1);
+    v.nl_tp = d_nl_tp;
+    v.nl_mask = d_nl_mask;
+    v.nl_pref = d_nl_pref;
+    v.nl_entries = d_nl_entries;
+    v.nl_xref = d_nl_xref;
+    v.nl_flag = d_nl_flag;
+    if (arena) {
+      alloc_work(arena_items, arena_tiles);
+      // what the hipMalloc path zeroes per buffer (dalloc), cleared at once
+      HIPCHECK(hipMemset(arena, 0, arena_used));
+    }
+    HIPCHECK(hipMemset(v.slot_of, 0xFF, std::max<int64_t>(cap_atoms, 1) * sizeof(int32_t)));
+    mark_free_entries();
+    HIPCHECK(hipMemset(d_nl_tp, 0xFF, std::max<int64_t>(cap_tp, 1) * sizeof(prx::TilePairInfo)));
+    if (arena) return;
     HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     HIPCHECK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
     HIPCHECK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
@@ -322,11 +427,13 @@ struct prx_engine {
 
   void release_device() {
     drop_graphs();
+    if (arena) return;  // nothing here was allocated
     void* ptrs[] = {v.pos, v.mom, v.force, v.born4, v.nb4, v.fin4, v.sqrt_m, v.slot_of, v.gb4, v.fpair, d_part, d_tiles,
                     v.e_atom, v.atom_step, v.bonds, v.angles, v.tors, v.cmaps, v.excs, v.e_bond, v.e_angle,
                     v.e_tors, v.e_cmap, v.e_exc, d_coeff, d_excl, v.slots, d_ranges, d_slot_list,
                     d_energy_tmp, d_map, d_items, d_e_items, v.rec, v.atom_rec, d_rec_list, v.warn,
-                    d_tbox, d_item_cls};
+                    d_tbox, d_item_cls, d_eval_pos, d_eval_force, d_nl_tp, d_nl_mask, d_nl_pref,
+                    d_nl_entries, d_nl_xref, d_nl_flag, d_nl_row, d_nl_count, d_nl_col, d_nl_colitems, d_bar};
     for (void* p : ptrs)
       if (p) hipFree(p);
     if (ev_fork) hipEventDestroy(ev_fork);
@@ -360,11 +467,19 @@ struct prx_engine {
     const int64_t n_coeff = d.n_cmap ? (int64_t)d.n_cmap_maps * d.cmap_grid * d.cmap_grid * 16 : 0;
     const int64_t n_xt = (int64_t)plan.tiles.size();
     const int64_t n_rl = count_records(d);
+    // A gb_cut system takes its pair work from a neighbour list when the
+    // engine has room for its tile-pair records and entries (pools keep
+    // none: their systems sweep all tile pairs, the same results).
+    int64_t n_tp = 0, n_ent = 0, n_nl_items = 0;
+    if (uses_nl(d)) prx::nl_sizes(d.n_atoms, plan, &n_tp, &n_ent, &n_nl_items);
+    s.nl = uses_nl(d) && n_tp <= a_tp.cap() && n_ent <= a_ent.cap() && a_tp.cap() > 0;
+    if (!s.nl) n_tp = n_ent = 0;
     struct Want { RangeAlloc* a; int64_t n; int64_t* b; };
     int64_t a0 = 0;
     Want w[] = {{&a_atoms, s.n, &a0}, {&a_bond, n_bond, &s.bond_b}, {&a_ang, n_ang, &s.ang_b},
                 {&a_tors, n_tors, &s.tors_b}, {&a_cmap, n_cmap, &s.cmap_b}, {&a_exc, n_exc, &s.exc_b},
-                {&a_coeff, n_coeff, &s.coeff_b}, {&a_xt, n_xt, &s.xt_b}, {&a_rl, n_rl, &s.rl_b}};
+                {&a_coeff, n_coeff, &s.coeff_b}, {&a_xt, n_xt, &s.xt_b}, {&a_rl, n_rl, &s.rl_b},
+                {&a_tp, n_tp, &s.tp_b}, {&a_ent, n_ent, &s.ent_b}};
     int got = 0;
     for (auto& x : w) {
       const int64_t b = x.a->alloc(x.n);
@@ -377,9 +492,30 @@ struct prx_engine {
     }
     s.a0 = (int32_t)a0;
     s.bond_n = n_bond; s.ang_n = n_ang; s.tors_n = n_tors; s.cmap_n = n_cmap; s.exc_n = n_exc;
-    s.coeff_n = n_coeff; s.xt_n = n_xt; s.rl_n = n_rl;
+    s.coeff_n = n_coeff; s.xt_n = n_xt; s.rl_n = n_rl; s.tp_n = n_tp; s.ent_n = n_ent;
     upload(d, sid, s, plan);
-    prx::build_items(s.n, s.a0, sid, plan, (int)s.xt_b, s.items);
+    if (s.nl) {
+      prx::NlPlan np;
+      prx::build_items_nl(s.n, s.a0, sid, plan, (int)s.xt_b, s.ent_b, np);
+      s.items = std::move(np.items);
+      s.nl_rows = std::move(np.rows);
+      s.nl_cols = std::move(np.cols);
+      if (!np.tps.empty())
+        HIPCHECK(hipMemcpy(d_nl_tp + s.tp_b, np.tps.data(), np.tps.size() * sizeof(prx::TilePairInfo),
+                           hipMemcpyHostToDevice));
+      prx::SlotConst& c = s.c;
+      c.nl = 1;
+      c.tp_base = (int32_t)s.tp_b;
+      const double rl = d.cutoff + nl_skin;
+      c.nl_cut2 = (float)(rl * rl * 1.00001);
+      c.skin_half2 = (float)(0.25 * nl_skin * nl_skin);
+      push_const(sid, c);
+      // the first evaluation builds the list
+      const int32_t one = 1;
+      HIPCHECK(hipMemcpy(d_nl_flag + sid, &one, sizeof(one), hipMemcpyHostToDevice));
+    } else {
+      prx::build_items(s.n, s.a0, sid, plan, (int)s.xt_b, s.items);
+    }
     s.live = true;
     slots[sid] = std::move(s);
     order.push_back(sid);
@@ -562,7 +698,7 @@ struct prx_engine {
     c.sqrt_kT = (float)std::sqrt(kT);
     c.caps = p.safety_caps ? 1 : 0;
     c.seed = p.seed;
-    HIPCHECK(hipMemset(v.warn + (size_t)sid * 4, 0, 4 * sizeof(unsigned int)));
+    HIPCHECK(hipMemset(v.warn + (size_t)sid * prx::kWarnStride, 0, prx::kWarnStride * sizeof(unsigned int)));
     slots[sid].has_params = true;
     push_const(sid, c);
   }
@@ -586,6 +722,9 @@ struct prx_engine {
     a_coeff.release(s.coeff_b, s.coeff_n);
     a_xt.release(s.xt_b, s.xt_n);
     a_rl.release(s.rl_b, s.rl_n);
+    if (s.tp_n) HIPCHECK(hipMemset(d_nl_tp + s.tp_b, 0xFF, s.tp_n * sizeof(prx::TilePairInfo)));
+    a_tp.release(s.tp_b, s.tp_n);
+    a_ent.release(s.ent_b, s.ent_n);
     s = Slot();
     order.erase(std::remove(order.begin(), order.end(), sid), order.end());
     dirty = true;
@@ -598,7 +737,11 @@ struct prx_engine {
     std::vector<int32_t> ranges;
     std::vector<int32_t> map;
     std::vector<int4> tiles;
+    std::vector<int4> nl_row;
+    std::vector<int2> nl_col;
+    std::vector<int32_t> nl_colitems;
     offsets.assign(1, 0);
+    has_nl = false;
     for (int32_t sid : order) {
       Slot& s = slots[sid];
       const int32_t ib = (int32_t)items.size();
@@ -606,9 +749,23 @@ struct prx_engine {
       s.c.tile0 = (int32_t)tiles.size();
       push_const(sid, s.c);
       const int nT = (s.n + prx::kTile - 1) / prx::kTile;
-      for (int T = 0; T < nT; ++T)
+      for (int T = 0; T < nT; ++T) {
         tiles.push_back(make_int4(s.a0 + T * prx::kTile, std::min(prx::kTile, s.n - T * prx::kTile), sid, T));
+        if (s.nl) {
+          const int4 r = s.nl_rows[T];
+          nl_row.push_back(make_int4(ib + r.x, (int)s.ent_b + r.y, ib + r.z, ib + r.w));
+          nl_col.push_back(make_int2((int)nl_colitems.size(), (int)s.nl_cols[T].size()));
+          for (int32_t k : s.nl_cols[T]) nl_colitems.push_back(ib + k);
+        } else {
+          nl_row.push_back(make_int4(0, 0, 0, 0));
+          nl_col.push_back(make_int2(0, 0));
+        }
+      }
+      has_nl = has_nl || s.nl;
       items.insert(items.end(), s.items.begin(), s.items.end());
+      if (s.nl)  // chunk items name their row's global tile
+        for (size_t k = items.size() - s.items.size(); k < items.size(); ++k)
+          if (prx::item_is_chunk(items[k])) items[k].excl += s.c.tile0;
       const int32_t r[14] = {ib, (int32_t)items.size(), s.a0, s.n, (int32_t)s.bond_b, (int32_t)(s.bond_b + s.bond_n),
                              (int32_t)s.ang_b, (int32_t)(s.ang_b + s.ang_n), (int32_t)s.tors_b,
                              (int32_t)(s.tors_b + s.tors_n), (int32_t)s.cmap_b, (int32_t)(s.cmap_b + s.cmap_n),
@@ -618,6 +775,7 @@ struct prx_engine {
       offsets.push_back(offsets.back() + s.n);
     }
     if ((int64_t)items.size() > cap_items) {
+      if (arena) throw std::logic_error("arena engine: work list larger than sized");
       if (d_items) HIPCHECK(hipFree(d_items));
       if (d_e_items) HIPCHECK(hipFree(d_e_items));
       if (d_part) HIPCHECK(hipFree(d_part));
@@ -629,12 +787,32 @@ struct prx_engine {
       d_item_cls = dalloc<int32_t>(cap_items);
     }
     if ((int64_t)tiles.size() > cap_tiles) {
-      if (d_tiles) HIPCHECK(hipFree(d_tiles));
-      if (d_tbox) HIPCHECK(hipFree(d_tbox));
+      if (arena) throw std::logic_error("arena engine: tile list larger than sized");
+      void* old[] = {d_tiles, d_tbox, d_nl_row, d_nl_count, d_nl_col};
+      for (void* p : old)
+        if (p) HIPCHECK(hipFree(p));
       cap_tiles = std::max<int64_t>((int64_t)tiles.size(), cap_tiles * 3 / 2);
       d_tiles = dalloc<int4>(cap_tiles);
       d_tbox = dalloc<float4>(cap_tiles * 2);
+      d_nl_row = dalloc<int4>(cap_tiles);
+      d_nl_count = dalloc<int32_t>(cap_tiles);
+      d_nl_col = dalloc<int2>(cap_tiles);
     }
+    if ((int64_t)nl_colitems.size() > cap_colitems) {
+      if (arena) throw std::logic_error("arena engine: column lists larger than sized");
+      if (d_nl_colitems) HIPCHECK(hipFree(d_nl_colitems));
+      cap_colitems = std::max<int64_t>((int64_t)nl_colitems.size(), cap_colitems * 3 / 2);
+      d_nl_colitems = dalloc<int32_t>(cap_colitems);
+    }
+    if (!nl_row.empty()) {
+      HIPCHECK(hipMemcpy(d_nl_row, nl_row.data(), nl_row.size() * sizeof(int4), hipMemcpyHostToDevice));
+      HIPCHECK(hipMemcpy(d_nl_col, nl_col.data(), nl_col.size() * sizeof(int2), hipMemcpyHostToDevice));
+      // every list is rebuilt at the next evaluation (tiles may have moved in the arrays)
+      HIPCHECK(hipMemset(d_nl_count, 0, nl_row.size() * sizeof(int32_t)));
+    }
+    if (!nl_colitems.empty())
+      HIPCHECK(hipMemcpy(d_nl_colitems, nl_colitems.data(), nl_colitems.size() * sizeof(int32_t),
+                         hipMemcpyHostToDevice));
     if (!tiles.empty())
       HIPCHECK(hipMemcpy(d_tiles, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice));
     if (!items.empty())
@@ -650,6 +828,11 @@ struct prx_engine {
     v.n_tiles = (int32_t)tiles.size();
     v.tbox = d_tbox;
     v.item_cls = d_item_cls;
+    v.nl_row = d_nl_row;
+    v.nl_count = d_nl_count;
+    v.nl_col = d_nl_col;
+    v.nl_colitems = d_nl_colitems;
+    v.nl_tp_hw = (int32_t)a_tp.high_water();
     v.atom_hw = (int32_t)a_atoms.high_water();
     v.bond_hw = (int32_t)a_bond.high_water();
     v.angle_hw = (int32_t)a_ang.high_water();
@@ -661,7 +844,16 @@ struct prx_engine {
   }
 
   int32_t total_atoms() const { return offsets.back(); }
-  bool fork_bonded() const { return PRX_FORK_BONDED && total_atoms() >= PRX_FORK_MIN_ATOMS; }
+  bool fork_bonded() const { return PRX_FORK_BONDED && total_atoms() >= fork_min_atoms; }
+
+  // The engine's view with the evaluation buffers in place of the
+  // integrator's positions and forces (prx_energy_forces).
+  prx::DeviceView eval_view() const {
+    prx::DeviceView e = v;
+    e.pos = d_eval_pos;
+    e.force = d_eval_force;
+    return e;
+  }
 
   // ---- evaluation -------------------------------------------------------
   // Everything of one force evaluation up to the chain gather. The bonded
@@ -672,32 +864,36 @@ struct prx_engine {
   // branch).
   // boxes = false when the previous chain gather (mode 1) already wrote the
   // tile boxes of the current positions.
-  void force_passes(bool energy, hipStream_t s, bool boxes = true) {
+  void force_passes(const prx::DeviceView& vv, bool energy, hipStream_t s, bool boxes = true) {
     const bool fork = fork_bonded();
     if (fork) {
       HIPCHECK(hipEventRecord(ev_fork, s));
       HIPCHECK(hipStreamWaitEvent(side, ev_fork, 0));
-      prx::launch_bonded(v, energy, side);
+      prx::launch_bonded(vv, energy, side);
       HIPCHECK(hipEventRecord(ev_join, side));
     }
-    if (boxes) prx::launch_tile_boxes(v, s);
-    prx::launch_born_pairs(v, s);
-    prx::launch_born_gather(v, s);
-    prx::launch_force_pairs(v, energy, s);
-    prx::launch_force_gather(v, energy, s);
+    if (boxes) prx::launch_tile_boxes(vv, s);
+    // every list is rebuilt where the boxes are (eager evaluations, the first
+    // step of a graph launch); later steps rebuild by the skin rule
+    if (has_nl) prx::launch_nl_build(vv, d_ranges, (int)order.size(), boxes, s);
+    prx::launch_born_pairs(vv, s);
+    prx::launch_born_gather(vv, s);
+    prx::launch_force_pairs(vv, energy, s);
+    prx::launch_force_gather(vv, energy, s);
     if (fork) {
-      prx::launch_chain_pairs(v, s);
+      prx::launch_chain_pairs(vv, s);
       HIPCHECK(hipStreamWaitEvent(s, ev_join, 0));
     } else {
-      prx::launch_bonded(v, energy, s);
-      prx::launch_chain_pairs(v, s);
+      prx::launch_bonded(vv, energy, s);
+      prx::launch_chain_pairs(vv, s);
     }
   }
 
-  void evaluate(bool energy, hipStream_t s) {
-    force_passes(energy, s);
-    prx::launch_chain_gather(v, -1, s);
+  void evaluate(const prx::DeviceView& vv, bool energy, hipStream_t s) {
+    force_passes(vv, energy, s);
+    prx::launch_chain_gather(vv, -1, s);
   }
+  void evaluate(bool energy, hipStream_t s) { evaluate(v, energy, s); }
 
   void drop_graphs() {
     for (auto& g : graphs) hipGraphExecDestroy(g.second);
@@ -714,7 +910,7 @@ struct prx_engine {
       prx::launch_step_begin(v, stream);
       for (int k = 0; k < n_steps; ++k) {
         // after the first step, the previous chain gather wrote the boxes
-        force_passes(false, stream, k == 0);
+        force_passes(v, false, stream, k == 0);
         prx::launch_chain_gather(v, k + 1 == n_steps ? 2 : 1, stream);
       }
       HIPCHECK(hipGetLastError());
@@ -730,9 +926,62 @@ struct prx_engine {
     return exec;
   }
 
+  // Work items up to which the auto step path runs the persistent kernel.
+  static constexpr int64_t kPersistAutoItems = 16384;
+  // Steps per persistent launch: long enough to amortise the launch, short
+  // enough that one launch stays well under a second on the largest engine.
+  static constexpr int kPersistChunk = 1000;
+
+  bool use_persistent() {
+    if (step_mode == 1) return false;
+    if (persist_blocks < 0) persist_blocks = prx::persist_max_blocks();
+    if (step_mode == 2) {
+      if (persist_blocks <= 0) throw StateError("the persistent step kernel needs cooperative launches");
+      return true;
+    }
+    return persist_blocks > 0 && v.n_items <= kPersistAutoItems;
+  }
+
+  // The persistent kernel: at most one item per wave and one tile per block
+  // in each pass, within the co-resident grid.
+  void steps_persistent(int n_steps, hipStream_t s) {
+    const int64_t want = std::max<int64_t>({(int64_t)v.n_tiles, ((int64_t)v.n_items + 3) / 4, 1});
+    const int blocks = (int)std::min<int64_t>(want, persist_blocks);
+    for (int left = n_steps; left > 0;) {
+      const int k = std::min(left, kPersistChunk);
+      HIPCHECK(prx::launch_persist_steps(v, d_ranges, (int)order.size(), k, has_nl, d_bar, blocks, s));
+      left -= k;
+    }
+    // a barrier that timed out leaves the error word set: report it
+    prx::GridBarrier b{};
+    HIPCHECK(hipMemcpyAsync(&b, d_bar, sizeof(b), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    if (b.error) throw HipError("persistent step kernel: a grid barrier timed out");
+  }
+
   void steps(int n_steps, hipStream_t s) {
     rebuild();
     if (order.empty() || n_steps <= 0) return;
+    if (use_persistent()) {
+      hipEvent_t ev;
+      HIPCHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+      if (s != stream) {
+        HIPCHECK(hipEventRecord(ev, s));
+        HIPCHECK(hipStreamWaitEvent(stream, ev, 0));
+      }
+      try {
+        steps_persistent(n_steps, stream);
+      } catch (...) {
+        hipEventDestroy(ev);
+        throw;
+      }
+      if (s != stream) {
+        HIPCHECK(hipEventRecord(ev, stream));
+        HIPCHECK(hipStreamWaitEvent(s, ev, 0));
+      }
+      HIPCHECK(hipEventDestroy(ev));
+      return;
+    }
     // Graphs are captured on the engine stream; order them after the caller's stream.
     const int chunk = std::min(n_steps, 50);
     hipEvent_t ev;
@@ -795,6 +1044,8 @@ int32_t prx_abi_version(void) { return PRX_ABI_VERSION; }
 int prx_system_create(const prx_system_desc* descs, int32_t n_systems, prx_engine** out) {
   return guard([&] {
     need(descs != nullptr && out != nullptr && n_systems > 0, "prx_system_create: bad arguments");
+    // a work item keeps its system's slot in 16 bits (PairItem.meta)
+    need(n_systems < 65536, "prx_system_create: at most 65535 systems per engine");
     *out = nullptr;
     auto eng = std::make_unique<prx_engine>();
     for (int k = 0; k < n_systems; ++k) {
@@ -811,6 +1062,10 @@ int prx_system_create(const prx_system_desc* descs, int32_t n_systems, prx_engin
       eng->cap_coeff += d.n_cmap ? (int64_t)d.n_cmap_maps * d.cmap_grid * d.cmap_grid * 16 : 0;
       eng->cap_xt += (int64_t)plan.tiles.size();
       eng->cap_rl += count_records(d);
+      int64_t n_tp = 0, n_ent = 0;
+      plan_items(d, plan, &n_tp, &n_ent);
+      eng->cap_tp += n_tp;
+      eng->cap_ent += n_ent;
     }
     eng->cap_slots = n_systems;
     eng->a_atoms = RangeAlloc(eng->cap_atoms);
@@ -822,6 +1077,8 @@ int prx_system_create(const prx_system_desc* descs, int32_t n_systems, prx_engin
     eng->a_coeff = RangeAlloc(eng->cap_coeff);
     eng->a_xt = RangeAlloc(eng->cap_xt);
     eng->a_rl = RangeAlloc(std::max<int64_t>(eng->cap_rl, 1));
+    eng->a_tp = RangeAlloc(eng->cap_tp);
+    eng->a_ent = RangeAlloc(eng->cap_ent);
     try {
       eng->alloc_device();
       for (int k = 0; k < n_systems; ++k) eng->admit(descs[k]);
@@ -898,12 +1155,13 @@ int prx_energy_forces(prx_engine* eng, const float* positions, float* forces, do
     eng->rebuild();
     hipStream_t s = eng->pick(stream);
     const int n = eng->total_atoms();
-    prx::launch_copy_in_positions(eng->v, positions, eng->d_map, n, s);
-    eng->evaluate(energies != nullptr, s);
+    const prx::DeviceView ev = eng->eval_view();
+    prx::launch_copy_in_positions(ev, positions, eng->d_map, n, s);
+    eng->evaluate(ev, energies != nullptr, s);
     if (energies)
-      prx::launch_energy_reduce(eng->v, eng->d_ranges, (int)eng->order.size(), energies, s);
-    if (forces) prx::launch_copy_out(eng->v, nullptr, nullptr, forces, eng->d_map, n, s);
-    if (born_radii) prx::launch_born_out(eng->v, born_radii, eng->d_map, n, s);
+      prx::launch_energy_reduce(ev, eng->d_ranges, (int)eng->order.size(), energies, s);
+    if (forces) prx::launch_copy_out(ev, nullptr, nullptr, forces, eng->d_map, n, s);
+    if (born_radii) prx::launch_born_out(ev, born_radii, eng->d_map, n, s);
     HIPCHECK(hipGetLastError());
   });
 }
@@ -915,6 +1173,10 @@ int prx_integrator_init(prx_engine* eng, const float* positions, const float* mo
     eng->rebuild();
     hipStream_t s = eng->pick(stream);
     const int n = eng->total_atoms();
+    // The slot constants are rewritten with synchronous copies: nothing queued
+    // on either stream (a step graph, the caller's work) may still read them.
+    HIPCHECK(hipStreamSynchronize(eng->stream));
+    HIPCHECK(hipStreamSynchronize(s));
     for (size_t k = 0; k < eng->order.size(); ++k) eng->set_params(eng->order[k], params[k]);
     prx::launch_copy_in_positions(eng->v, positions, eng->d_map, n, s);
     if (momenta) {
@@ -1010,8 +1272,78 @@ int prx_integrator_warnings(prx_engine* eng, int32_t system, int64_t* counts) {
     need(eng && counts && system >= 0 && system < (int32_t)eng->order.size(), "prx_integrator_warnings: bad arguments");
     unsigned int w[4] = {0, 0, 0, 0};
     HIPCHECK(hipDeviceSynchronize());
-    HIPCHECK(hipMemcpy(w, eng->v.warn + (size_t)eng->order[system] * 4, sizeof(w), hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(w, eng->v.warn + (size_t)eng->order[system] * prx::kWarnStride, sizeof(w),
+                       hipMemcpyDeviceToHost));
     for (int k = 0; k < 4; ++k) counts[k] = w[k];
+  });
+}
+
+int prx_integrator_set_step_counts(prx_engine* eng, const int64_t* steps, void* stream) {
+  return guard([&] {
+    need(eng && steps, "prx_integrator_set_step_counts: bad arguments");
+    if (!eng->initialized) throw StateError("prx_integrator_set_step_counts before prx_integrator_init");
+    eng->rebuild();
+    hipStream_t s = eng->pick(stream);
+    for (size_t k = 0; k < eng->order.size(); ++k)
+      need(steps[k] >= 0 && steps[k] <= (int64_t)UINT32_MAX,
+           "prx_integrator_set_step_counts: step counts must lie in [0, 2^32)");
+    for (size_t k = 0; k < eng->order.size(); ++k) {
+      const Slot& sl = eng->slots[eng->order[k]];
+      prx::launch_fill_u32(eng->v.atom_step + sl.a0, (uint32_t)steps[k], sl.n, s);
+    }
+    HIPCHECK(hipGetLastError());
+  });
+}
+
+int prx_integrator_health(prx_engine* eng, int64_t* nonfinite, void* stream) {
+  return guard([&] {
+    need(eng && nonfinite, "prx_integrator_health: bad arguments");
+    hipStream_t s = eng->pick(stream);
+    HIPCHECK(hipStreamSynchronize(s));
+    HIPCHECK(hipStreamSynchronize(eng->stream));
+    std::vector<unsigned int> w((size_t)eng->cap_slots * prx::kWarnStride);
+    HIPCHECK(hipMemcpy(w.data(), eng->v.warn, w.size() * sizeof(unsigned int), hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < eng->order.size(); ++k)
+      nonfinite[k] = w[(size_t)eng->order[k] * prx::kWarnStride + prx::kWarnNonFinite];
+  });
+}
+
+int prx_engine_set_nl_skin(prx_engine* eng, double skin) {
+  return guard([&] {
+    need(eng && skin >= 0.0 && skin < 1e3, "prx_engine_set_nl_skin: skin must lie in [0, 1000) A");
+    HIPCHECK(hipStreamSynchronize(eng->stream));
+    eng->nl_skin = skin;
+    for (int32_t sid : eng->order) {
+      Slot& sl = eng->slots[sid];
+      if (!sl.nl) continue;
+      const double rl = std::sqrt((double)sl.c.cutoff2) + skin;
+      sl.c.nl_cut2 = (float)(rl * rl * 1.00001);
+      sl.c.skin_half2 = (float)(0.25 * skin * skin);
+      eng->push_const(sid, sl.c);
+    }
+  });
+}
+
+int prx_engine_set_step_mode(prx_engine* eng, int32_t mode) {
+  return guard([&] {
+    need(eng && mode >= 0 && mode <= 2, "prx_engine_set_step_mode: mode is 0 (auto), 1 (graphs) or 2 (persistent)");
+    if (mode == 2) {
+      if (eng->persist_blocks < 0) eng->persist_blocks = prx::persist_max_blocks();
+      if (eng->persist_blocks <= 0) throw StateError("the persistent step kernel needs cooperative launches");
+    }
+    HIPCHECK(hipStreamSynchronize(eng->stream));
+    eng->step_mode = mode;
+  });
+}
+
+int prx_engine_set_fork_min_atoms(prx_engine* eng, int64_t min_atoms) {
+  return guard([&] {
+    need(eng && min_atoms >= 0, "prx_engine_set_fork_min_atoms: bad arguments");
+    if (eng->fork_min_atoms == min_atoms) return;
+    // captured graphs hold the old shape: wait for them, then drop them
+    HIPCHECK(hipStreamSynchronize(eng->stream));
+    eng->drop_graphs();
+    eng->fork_min_atoms = min_atoms;
   });
 }
 
@@ -1099,8 +1431,12 @@ int prx_pool_energy(prx_engine* pool, int32_t slot, double* energies_host, void*
     pool->rebuild();
     hipStream_t s = pool->pick(stream);
     HIPCHECK(hipStreamSynchronize(pool->stream));
-    pool->evaluate(true, s);
-    prx::launch_energy_reduce(pool->v, pool->d_ranges, (int)pool->order.size(), pool->d_energy_tmp, s);
+    // evaluated in the evaluation buffers at a copy of the positions: the
+    // integrator's stored forces stay those of its own force-only passes
+    const prx::DeviceView ev = pool->eval_view();
+    HIPCHECK(hipMemcpyAsync(ev.pos, pool->v.pos, (size_t)pool->v.atom_hw * sizeof(float4), hipMemcpyDeviceToDevice, s));
+    pool->evaluate(ev, true, s);
+    prx::launch_energy_reduce(ev, pool->d_ranges, (int)pool->order.size(), pool->d_energy_tmp, s);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(s));
     const auto it = std::find(pool->order.begin(), pool->order.end(), slot);
@@ -1137,6 +1473,7 @@ int prx_profile_passes(prx_engine* eng, int32_t n_reps, double* ms) {
       ms[k] = t;
     };
     prx::launch_tile_boxes(eng->v, s);
+    if (eng->has_nl) prx::launch_nl_build(eng->v, eng->d_ranges, (int)eng->order.size(), true, s);
     timed(0, [&] { prx::launch_born_pairs(eng->v, s); });
     timed(1, [&] { prx::launch_born_gather(eng->v, s); });
     timed(2, [&] { prx::launch_force_pairs(eng->v, false, s); });
@@ -1198,6 +1535,7 @@ int prx_profile_steps(prx_engine* eng, int32_t n_steps, double* ms) {
             HIPCHECK(hipEventRecord(eng->ev_join, eng->side));
           }
           if (k == 0) prx::launch_tile_boxes(v, s);
+          if (eng->has_nl) prx::launch_nl_build(v, eng->d_ranges, (int)eng->order.size(), k == 0, s);
           timed(0, s, [&] { prx::launch_born_pairs(v, s); });
           timed(1, s, [&] { prx::launch_born_gather(v, s); });
           timed(2, s, [&] { prx::launch_force_pairs(v, false, s); });
@@ -1255,8 +1593,8 @@ int prx_plan_tiles(const prx_system_desc* desc, int64_t* n_items, int64_t* n_exc
     validate(*desc);
     prx::ExclPlan plan;
     prx::build_exclusions(*desc, plan);
-    const int64_t nT = (desc->n_atoms + prx::kTile - 1) / prx::kTile;
-    *n_items = nT * (nT + 1) / 2;
+    int64_t n_tp = 0, n_ent = 0;
+    *n_items = plan_items(*desc, plan, &n_tp, &n_ent);
     *n_excl_tiles = (int64_t)plan.tiles.size();
   });
 }

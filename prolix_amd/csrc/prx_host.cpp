@@ -234,4 +234,92 @@ void build_items(int n_atoms, int a0, int slot, const ExclPlan& plan, int excl_b
     }
 }
 
+namespace {
+// (I, J) -> exclusion tile index of the plan, for I <= J.
+std::map<std::pair<int, int>, int> excl_tile_index(const ExclPlan& plan) {
+  std::map<std::pair<int, int>, int> tix;
+  for (size_t t = 0; t < plan.tiles.size(); ++t) tix[plan.tiles[t]] = (int)t;
+  return tix;
+}
+}  // namespace
+
+void nl_sizes(int n_atoms, const ExclPlan& plan, int64_t* n_tp, int64_t* n_entries, int64_t* n_items) {
+  const int nT = (n_atoms + kTile - 1) / kTile;
+  const auto tix = excl_tile_index(plan);
+  int64_t ent = 0, items = nT, chunks = 0;
+  for (int I = 0; I < nT; ++I) {
+    int64_t cap = 0;
+    for (int J = I + 1; J < nT; ++J) {
+      if (tix.count({I, J})) {
+        ++items;
+        continue;
+      }
+      cap += std::min(kTile, n_atoms - J * kTile);
+    }
+    ent += cap;
+    chunks += (cap + kTile - 1) / kTile;
+  }
+  *n_tp = (int64_t)nT * (nT - 1) / 2;
+  *n_entries = ent;
+  *n_items = items + chunks;
+}
+
+void build_items_nl(int n_atoms, int a0, int slot, const ExclPlan& plan, int excl_base, int64_t ent_b,
+                    NlPlan& out) {
+  const int nT = (n_atoms + kTile - 1) / kTile;
+  const auto tix = excl_tile_index(plan);
+  out.items.clear();
+  out.rows.assign(nT, make_int4(0, 0, 0, 0));
+  out.cols.assign(nT, {});
+  out.tps.clear();
+  auto n_of = [&](int T) { return std::min(kTile, n_atoms - T * kTile); };
+  // regular items, row-major: the diagonal tile and the tile pairs with exclusions
+  for (int I = 0; I < nT; ++I) {
+    out.rows[I].z = (int)out.items.size();
+    for (int J = I; J < nT; ++J) {
+      auto it = tix.find({I, J});
+      if (J != I && it == tix.end()) continue;
+      int flags = (I == J) ? kFlagDiag : 0;
+      int ex = -1;
+      if (it != tix.end()) {
+        flags |= kFlagExcl;
+        ex = excl_base + it->second;
+      }
+      PairItem p;
+      p.i0 = a0 + I * kTile;
+      p.j0 = a0 + J * kTile;
+      p.meta = item_meta(n_of(I), n_of(J), flags, slot);
+      p.excl = ex;
+      out.cols[J].push_back((int32_t)out.items.size());
+      out.items.push_back(p);
+    }
+    out.rows[I].w = (int)out.items.size();
+  }
+  // chunk items of each row's list, and the tile-pair records
+  int64_t ent = 0;
+  for (int I = 0; I < nT; ++I) {
+    int64_t cap = 0;
+    for (int J = I + 1; J < nT; ++J) {
+      const bool ex = tix.count({I, J}) != 0;
+      TilePairInfo tp;
+      tp.ij = I | (J << 16);
+      tp.slot = ex ? ~slot : slot;
+      out.tps.push_back(tp);
+      if (!ex) cap += n_of(J);
+    }
+    out.rows[I].x = (int)out.items.size();
+    out.rows[I].y = (int)ent;
+    for (int64_t c = 0; c < cap; c += kTile) {
+      PairItem p;
+      p.i0 = a0 + I * kTile;
+      p.j0 = (int32_t)(-1 - (ent_b + ent + c));
+      p.meta = item_meta(n_of(I), kTile, 0, slot);
+      p.excl = I;
+      out.items.push_back(p);
+    }
+    ent += cap;
+  }
+  out.n_entries = ent;
+}
+
 }  // namespace prx

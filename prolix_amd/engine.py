@@ -130,6 +130,41 @@ class GBEngine:
     def steps(self, n_steps: int):
         N.check(N.lib().prx_integrator_steps(self._h, int(n_steps), _stream(self.torch)), "prx_integrator_steps")
 
+    def set_step_counts(self, steps):
+        """Set each system's step count, which is the position of its Philox noise
+        stream (after ``init`` with saved momenta: resume a saved trajectory)."""
+        if np.ndim(steps) == 0:
+            steps = [int(steps)] * self.n_systems
+        if len(steps) != self.n_systems:
+            raise ValueError("one step count per system is required")
+        arr = (ctypes.c_int64 * self.n_systems)(*[int(x) for x in steps])
+        N.check(N.lib().prx_integrator_set_step_counts(self._h, arr, _stream(self.torch)),
+                "prx_integrator_set_step_counts")
+
+    def nonfinite_counts(self) -> np.ndarray:
+        """Per system: atom-steps since init that left a non-finite position or
+        momentum (0 for a healthy system)."""
+        arr = (ctypes.c_int64 * max(1, self.n_systems))()
+        N.check(N.lib().prx_integrator_health(self._h, arr, _stream(self.torch)), "prx_integrator_health")
+        return np.array(arr[: self.n_systems], np.int64)
+
+    def set_nl_skin(self, skin: float):
+        """Verlet skin (A) of the neighbour lists of gb_cut systems: 0 rebuilds
+        the list at every evaluation, skin > 0 when an atom moved > skin / 2."""
+        N.check(N.lib().prx_engine_set_nl_skin(self._h, float(skin)), "prx_engine_set_nl_skin")
+
+    STEP_MODES = {"auto": 0, "graph": 1, "persistent": 2}
+
+    def set_step_mode(self, mode: str):
+        """How steps run: "graph" (captured per-pass kernels), "persistent" (one
+        cooperative kernel per 1,000 steps, the same results bit for bit) or
+        "auto" (persistent for small engines)."""
+        N.check(N.lib().prx_engine_set_step_mode(self._h, self.STEP_MODES[mode]), "prx_engine_set_step_mode")
+
+    def set_fork_min_atoms(self, min_atoms: int):
+        """Run the bonded kernel beside the pair passes from this many atoms on."""
+        N.check(N.lib().prx_engine_set_fork_min_atoms(self._h, int(min_atoms)), "prx_engine_set_fork_min_atoms")
+
     def state(self):
         t = self.torch
         x = t.empty((self.n_atoms, 3), dtype=t.float32, device=self.device)
@@ -248,6 +283,14 @@ class GBPool:
         N.check(N.lib().prx_pool_energy(self._h, int(slot), e.ctypes.data_as(N.c_f64p), _stream(self.torch)),
                 "prx_pool_energy")
         return e
+
+    def nonfinite_counts(self) -> dict:
+        """Per live slot: atom-steps since admission that left a non-finite
+        position or momentum."""
+        live = self.live()
+        arr = (ctypes.c_int64 * max(1, len(live)))()
+        N.check(N.lib().prx_integrator_health(self._h, arr, _stream(self.torch)), "prx_integrator_health")
+        return {slot: int(arr[k]) for k, slot in enumerate(live)}
 
     def live(self):
         n = ctypes.c_int32()

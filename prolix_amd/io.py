@@ -78,6 +78,10 @@ class SimulationState:
     mass: Any = None
     potential_energy: Any = None
     kinetic_energy: Any = None
+    # The integrator's own float32 momenta (amu A per AKMA time unit), beside
+    # the velocities: with ``step`` they resume a trajectory bit for bit. An
+    # extra record key that prolix's reader (typing.py:706-728) skips.
+    momenta: Any = None
 
     @property
     def position(self):
@@ -92,7 +96,7 @@ class SimulationState:
             return np.asarray(x)
 
         return {k: cpu(getattr(self, k)) for k in ("positions", "velocities", "forces", "mass", "step", "time_ns",
-                                                     "potential_energy", "kinetic_energy")}
+                                                     "potential_energy", "kinetic_energy", "momenta")}
 
     def to_array_record(self) -> bytes:
         return packb({k: v for k, v in self.numpy().items() if v is not None})
@@ -102,7 +106,7 @@ class SimulationState:
         d = unpackb(packed)
         return cls(positions=d["positions"], velocities=d["velocities"], step=d["step"], time_ns=d["time_ns"],
                    forces=d.get("forces"), mass=d.get("mass"), potential_energy=d.get("potential_energy"),
-                   kinetic_energy=d.get("kinetic_energy"))
+                   kinetic_energy=d.get("kinetic_energy"), momenta=d.get("momenta"))
 
 
 class TrajectoryWriter:
@@ -189,8 +193,29 @@ def engine_states(engine, masses_amu, dt_fs: float, energies: bool = True) -> li
             positions=np.asarray(xs[k], np.float32), velocities=(ps[k] / mk[:, None]).astype(np.float32),
             step=np.int64(steps), time_ns=np.float64(steps * dt_fs * 1e-6), forces=np.asarray(fs[k], np.float32),
             mass=mk, potential_energy=None if e is None else np.float64(e[k, -1]),
-            kinetic_energy=None if ke is None else np.float64(ke[k])))
+            kinetic_energy=None if ke is None else np.float64(ke[k]), momenta=np.asarray(ps[k], np.float32)))
     return out
+
+
+def restore_engine(engine, states, params) -> None:
+    """Load one saved state per system into ``engine`` and continue from it.
+
+    Positions, the saved float32 momenta and the step count (the position of
+    each system's Philox noise stream) are restored, and the forces are
+    recomputed at the saved positions, so the continued trajectory equals an
+    uninterrupted one bit for bit (the resumed production of
+    batched_simulate.py:1300-1305). ``params``: one Langevin per system (or one
+    for all). States written without momenta cannot be resumed exactly and
+    raise ValueError.
+    """
+    if len(states) != engine.n_systems:
+        raise ValueError("one state per system is required")
+    if any(st.momenta is None for st in states):
+        raise ValueError("a state without the integrator's momenta cannot resume a trajectory exactly")
+    x = [np.asarray(st.positions, np.float32).reshape(-1, 3) for st in states]
+    p = [np.asarray(st.momenta, np.float32).reshape(-1, 3) for st in states]
+    engine.init(x, params, momenta=p)
+    engine.set_step_counts([int(np.asarray(st.step)) for st in states])
 
 
 DEFAULT_RECENTER_EVERY = 100  # steps between centre-of-mass re-centrings (batched_simulate.py:55)

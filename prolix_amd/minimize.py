@@ -193,8 +193,12 @@ FIRE_STAGES = ((10.0, 500), (10.0, 500), (100.0, 500), (1000.0, 1000), (None, 30
 
 
 def staged_fire(engine, positions, masses, stages=FIRE_STAGES):
-    """Dynamic dt as simulate.py:642-656, then the capped stages; returns packed positions."""
+    """Dynamic dt as simulate.py:642-656, then the capped stages; returns packed positions.
+    A non-finite starting energy raises RuntimeError first (simulate.py:571-582)."""
+    from prolix_amd.checks import check_initial_energy
+
     t = engine.torch
+    check_initial_energy(engine, positions)
     _, f0, _ = engine.energy_forces(positions, energies=False)
     max_grad = float(t.linalg.vector_norm(f0, dim=1).max())
     dt_start = float(np.clip(0.001 / (max_grad + 1e-8), 1e-7, 0.001))

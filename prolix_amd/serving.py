@@ -60,6 +60,7 @@ class Result:
     forces: np.ndarray
     energies: np.ndarray | None = None
     extra: dict = field(default_factory=dict)
+    error: str | None = None  # set when the system was evicted early (non-finite state)
 
     @property
     def latency(self) -> float:
@@ -123,21 +124,31 @@ def serve(requests, hbm_budget_bytes: float = 200e9, max_systems: int = 1024, ch
             n = len(v[0].system["charges"])
             pair_steps += chunk * n * (n - 1) / 2.0
             sys_steps += chunk
-        for slot in [s for s, v in live.items() if v[1] <= 0]:
+        # explosion check on the device counters: a system whose positions or
+        # momenta turned non-finite is evicted with an error result; the others
+        # never read its atoms, so they continue unchanged
+        failed = {slot for slot, c in pool.nonfinite_counts().items() if c > 0}
+        for slot in [s for s, v in live.items() if v[1] <= 0 or s in failed]:
             r, _, adm, cost = live.pop(slot)
             x, p, f, steps = pool.read(slot)
-            e = pool.energies(slot) if r.rid in sample else None
-            results.append(Result(r.rid, len(r.system["charges"]), steps, r.arrival, adm, now(), x, p, f, e))
+            err = None
+            if slot in failed:
+                err = f"non-finite positions or momenta after {steps} of {r.n_steps} steps"
+            e = pool.energies(slot) if r.rid in sample and err is None else None
+            results.append(Result(r.rid, len(r.system["charges"]), steps, r.arrival, adm, now(), x, p, f, e,
+                                  error=err))
             pool.release(slot)
             used -= cost
             if log:
                 log(f"request {r.rid}: {len(r.system['charges'])} atoms, {steps} steps, "
-                    f"latency {results[-1].latency:.2f}s, live {len(live)}, waiting {len(waiting)}")
+                    f"latency {results[-1].latency:.2f}s, live {len(live)}, waiting {len(waiting)}"
+                    + (f", evicted: {err}" if err else ""))
     wall = now()
     pool.close()
     lat = np.array([r.latency for r in results])
     stats = {
         "requests": len(results),
+        "failed": sum(r.error is not None for r in results),
         "wall_s": wall,
         "latency_p50_s": float(np.percentile(lat, 50)),
         "latency_p99_s": float(np.percentile(lat, 99)),

@@ -530,11 +530,13 @@ def _mix_member(args):
     return p
 
 
-def build_many(jobs, workers: int | None = None):
-    """Compact proteins for (n_atoms, seed) jobs, built in a forked process pool
-    (``workers`` processes, or the CPUs this process may use, at most 16); call
-    it before the process touches the GPU. Each system depends only on its own
-    seed, so the result is the same for any worker count."""
+def build_many(jobs, workers: int | None = None, start_method: str = "fork"):
+    """Compact proteins for (n_atoms, seed) jobs, built in a process pool
+    (``workers`` processes, or the CPUs this process may use, at most 16). With
+    the default "fork" start method call it before the process touches the GPU;
+    "spawn" starts fresh interpreters and is safe afterwards. Each system
+    depends only on its own seed, so the result is the same for any worker
+    count."""
     import os
 
     jobs = [(int(n), int(sd)) for n, sd in jobs]
@@ -546,7 +548,7 @@ def build_many(jobs, workers: int | None = None):
 
     # largest first, so the pool's tail is short
     order = sorted(range(len(jobs)), key=lambda k: -jobs[k][0])
-    with mp.get_context("fork").Pool(min(workers, len(jobs))) as pool:
+    with mp.get_context(start_method).Pool(min(workers, len(jobs))) as pool:
         built = pool.map(_mix_member, [jobs[k] for k in order], chunksize=1)
     systems = [None] * len(jobs)
     for k, p in zip(order, built):

@@ -108,9 +108,11 @@ def test_complete_list_equals_dense():
 
 def test_mixed_cutoff_modes_batch_equals_alone():
     """Systems with different cutoff modes share one pool; each result equals the
-    system evaluated alone, bitwise. (Forces of an energy evaluation are compared
-    with forces of an energy evaluation: the energy and the force-only variants
-    of the pair kernel are compiled separately and may round differently.)"""
+    system alone in a pool of its own, bitwise. (Pools keep no neighbour lists:
+    a gb_cut system sweeps its tile pairs there; an engine's list evaluation
+    agrees with that to rounding, tests/test_gpu_nl.py. The stored forces come
+    from the force-only variant of the pair kernel, energies from the energy
+    variant; the two are compiled separately and may round differently.)"""
     from prolix_amd.engine import GBEngine, GBPool, Langevin
 
     systems = [protein(n, 40 + k, radius=synth.compact_radius(n)) for k, n in enumerate((900, 600, 1300))]
@@ -122,11 +124,20 @@ def test_mixed_cutoff_modes_batch_equals_alone():
     got = [pool.read(sl) for sl in slots]
     pool.close()
     for k, (s, m) in enumerate(zip(systems, modes)):
-        eng = GBEngine(s, **m)
-        e, f, _ = eng.energy_forces(s["positions"], energies=True)
-        eng.close()
-        assert np.array_equal(got[k][2], f.cpu().numpy()), k
-        assert np.array_equal(ens[k], e.cpu().numpy()[0]), k
+        one = GBPool(max_atoms=len(s["charges"]), max_systems=1)
+        sl = one.admit(s, Langevin(1.0, 300.0, 1.0, seed=k), **m)
+        e = one.energies(sl)
+        f = one.read(sl)[2]
+        one.close()
+        assert np.array_equal(got[k][2], f), k
+        assert np.array_equal(ens[k], e), k
+        if not m.get("gb_cutoff"):  # without a list, an engine runs the same sweep
+            eng = GBEngine(s, **m)
+            e2, _, _ = eng.energy_forces(s["positions"], energies=True)
+            _, f2, _ = eng.energy_forces(s["positions"], energies=False)
+            eng.close()
+            assert np.array_equal(got[k][2], f2.cpu().numpy()), k
+            assert np.array_equal(ens[k], e2.cpu().numpy()[0]), k
 
 
 def test_large_gb_cutoff_20k():

@@ -1,0 +1,108 @@
+"""The device neighbour list of gb_cut systems (prx_nl.hip): per 64-atom tile
+the atoms of later tiles within cutoff + skin of its box, evaluated in
+chunks of 64 with the per-pair cutoff test.
+
+- the list engine equals the all-tile-pairs sweep of a serving pool (which
+  keeps no lists) to float32 rounding, and the float64 restatement to the
+  parity tolerances;
+- with a Verlet skin the list is rebuilt only when an atom moved more than
+  skin / 2 (jax_md's rule, physics/neighbor_list.py:155-172): after steps
+  that reused lists, the forces the step graph stored equal a fresh list's
+  to rounding, and the float64 restatement at those positions;
+- several list systems and a dense one packed in one engine give each
+  system's own result bit for bit (lists never cross systems).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import cpu as C
+from prolix_amd import synth
+from tests.helpers import protein, rel_rms
+
+pytestmark = pytest.mark.gpu
+
+E_TOL = 1e-6
+F_TOL = 1e-5
+
+
+def _e_scale(terms):
+    return max(abs(terms[-1]), 0.2 * float(np.sum(np.abs(terms[:-1]))), 1e-30)
+
+
+def _oracle(s, x, rc):
+    return C.CpuSystem(s, cutoff=rc, gb_cutoff=True, f32_cutoff=True).energy_forces(np.asarray(x, np.float64))
+
+
+def test_list_matches_all_tile_pairs_and_float64():
+    from prolix_amd.engine import GBEngine, GBPool, Langevin
+
+    s = protein(3000, 11, radius=synth.compact_radius(3000))
+    rc = 12.0
+    print("[gpu] list engine vs pool sweep, 3000 atoms, 12 A", flush=True)
+    eng = GBEngine(s, cutoff=rc, gb_cutoff=True)
+    e, f, _ = eng.energy_forces(s["positions"], energies=True)
+    e, f = e.cpu().numpy()[0], f.cpu().numpy()
+    eng.close()
+    pool = GBPool(max_atoms=4000, max_systems=4)
+    slot = pool.admit(s, Langevin(1.0, 300.0, 1.0, seed=1), cutoff=rc, gb_cutoff=True)
+    ep = pool.energies(slot)
+    _, _, fp, _ = pool.read(slot)
+    pool.close()
+    eo, fo = _oracle(s, s["positions"], rc)
+    print(f"[nl] list vs sweep: energy rel {abs(e[-1] - ep[-1]) / abs(ep[-1]):.2e}, force {rel_rms(f, fp):.2e}; "
+          f"vs float64: energy rel {abs(e[-1] - eo[-1]) / _e_scale(eo):.2e}, force {rel_rms(f, fo):.2e}", flush=True)
+    assert abs(e[-1] - ep[-1]) <= 1e-6 * _e_scale(ep)
+    assert rel_rms(f, fp) < 1e-6
+    assert abs(e[-1] - eo[-1]) / _e_scale(eo) < E_TOL
+    assert rel_rms(f, fo) < F_TOL
+    assert np.all(np.abs(e[:-1] - eo[:-1]) <= E_TOL * np.sum(np.abs(eo[:-1])) + 1e-9)
+
+
+@pytest.mark.parametrize("skin", [0.0, 2.0])
+def test_skin_reuse_keeps_forces_exact(skin):
+    from prolix_amd.engine import GBEngine, Langevin
+
+    s = protein(2500, 12, radius=synth.compact_radius(2500))
+    rc = 12.0
+    print(f"[gpu] list skin {skin}, 2500 atoms, 40 steps", flush=True)
+    eng = GBEngine(s, cutoff=rc, gb_cutoff=True)
+    eng.set_nl_skin(skin)
+    eng.init(s["positions"], Langevin(1.0, 300.0, 1.0, seed=4, safety_caps=True))
+    eng.steps(40)
+    x, _, f_step = eng.state()
+    e1, f1, _ = eng.energy_forces(x, energies=True)  # a fresh list at the final positions
+    x, f_step, f1, e1 = x.cpu().numpy(), f_step.cpu().numpy(), f1.cpu().numpy(), e1.cpu().numpy()[0]
+    warn = eng.warnings(0)
+    eng.close()
+    eo, fo = _oracle(s, x, rc)
+    print(f"[nl] skin {skin}: stored vs fresh force {rel_rms(f_step, f1):.2e}; vs float64 energy "
+          f"{abs(e1[-1] - eo[-1]) / _e_scale(eo):.2e} force {rel_rms(f1, fo):.2e}; caps {warn}", flush=True)
+    if warn[1] == 0:
+        assert rel_rms(f_step, f1) < 1e-6
+    assert abs(e1[-1] - eo[-1]) / _e_scale(eo) < E_TOL
+    assert rel_rms(f1, fo) < F_TOL
+
+
+def test_list_systems_packed_equal_alone():
+    from prolix_amd.engine import GBEngine, Langevin
+
+    rc = 10.0
+    systems = [protein(int(n), 30 + k, radius=synth.compact_radius(int(n))) for k, n in enumerate((900, 1500, 2100))]
+    print("[gpu] 3 list systems packed vs alone", flush=True)
+    eng = GBEngine(systems, cutoff=rc, gb_cutoff=True)
+    eng.init([s["positions"] for s in systems], [Langevin(1.0, 300.0, 1.0, seed=7 + k) for k in range(3)])
+    eng.steps(25)
+    x, p, f = (a.cpu().numpy() for a in eng.state())
+    off = eng.offsets
+    eng.close()
+    for k, s in enumerate(systems):
+        one = GBEngine(s, cutoff=rc, gb_cutoff=True)
+        one.init(s["positions"], Langevin(1.0, 300.0, 1.0, seed=7 + k))
+        one.steps(25)
+        xa, pa, fa = (a.cpu().numpy() for a in one.state())
+        one.close()
+        sl = slice(off[k], off[k + 1])
+        assert np.array_equal(xa, x[sl]) and np.array_equal(pa, p[sl]) and np.array_equal(fa, f[sl])

@@ -76,6 +76,29 @@ def test_plan_tiles_and_validation_errors():
         N.Desc({"charges": np.zeros(0)})
 
 
+def test_plan_tiles_of_a_neighbour_list_system():
+    """A gb_cut system's work: its diagonal tiles, the tile pairs with
+    exclusions, and ceil(c_I / 64) list chunks per row I, c_I = the atoms of
+    the exclusion-free tiles J > I (the list's capacity)."""
+    n = 1000
+    s = protein(n, 5)
+    nT = (n + 63) // 64
+    d = N.Desc(s, cutoff=12.0, gb_cutoff=True)
+    n_items, n_excl = N.plan_tiles(d)
+    # exclusion tile pairs (I < J), from the exclusion rows
+    excl = set()
+    idx = np.asarray(s["excl_indices"])
+    for i in range(n):
+        for j in idx[i]:
+            if j >= 0 and (i // 64) != (j // 64):
+                excl.add((min(i, j) // 64, max(i, j) // 64))
+    size = lambda J: min(64, n - 64 * J)
+    chunks = sum(-(-sum(size(J) for J in range(I + 1, nT) if (I, J) not in excl) // 64) for I in range(nT))
+    assert n_items == nT + len(excl) + chunks
+    # without the GB cut every tile pair is an item
+    assert N.plan_tiles(N.Desc(s, cutoff=12.0))[0] == nT * (nT + 1) // 2
+
+
 def test_more_than_three_exclusion_classes_rejected():
     s = protein(40, 1)
     sv = np.array(s["excl_scales_vdw"], np.float64)

@@ -56,11 +56,13 @@ def _state(rng, n=7, full=True):
         step=np.int64(1234), time_ns=np.float64(2.468e-3),
         forces=rng.normal(size=(n, 3)).astype(np.float32) if full else None,
         mass=rng.uniform(1, 16, n) if full else None,
-        potential_energy=np.float64(-1234.5) if full else None, kinetic_energy=12.25 if full else None)
+        potential_energy=np.float64(-1234.5) if full else None, kinetic_energy=12.25 if full else None,
+        momenta=rng.normal(size=(n, 3)).astype(np.float32) if full else None)
 
 
 def _same(a, b):
-    for k in ("positions", "velocities", "forces", "mass", "step", "time_ns", "potential_energy", "kinetic_energy"):
+    for k in ("positions", "velocities", "forces", "mass", "step", "time_ns", "potential_energy", "kinetic_energy",
+              "momenta"):
         x, y = getattr(a, k), getattr(b, k)
         if x is None:
             assert y is None, k
@@ -75,7 +77,7 @@ def test_state_round_trip(full):
     rec = st.to_array_record()
     keys = set(msgpack.unpackb(rec, raw=False))
     assert keys == ({"positions", "velocities", "step", "time_ns"} | (
-        {"forces", "mass", "potential_energy", "kinetic_energy"} if full else set()))
+        {"forces", "mass", "potential_energy", "kinetic_energy", "momenta"} if full else set()))
     back = io.SimulationState.from_array_record(rec)
     _same(st, back)
     assert back.position is back.positions
@@ -212,3 +214,17 @@ def test_streaming_into_trajectory_files(tmp_path):
     assert [int(st.step) for st in frames[1]] == [5, 10, 15, 20]
     assert frames[1][-1].time_ns == pytest.approx(20 * 2.0e-6)
     assert np.allclose(frames[1][-1].positions, eng.x[3:7], atol=1e-6)
+
+
+def test_restore_needs_the_integrator_momenta():
+    """A record without the float32 momenta (e.g. one written by prolix, whose
+    reader and writer know only the velocities) cannot resume bit for bit."""
+
+    class _Eng:
+        n_systems = 1
+
+    st = _state(np.random.default_rng(1), full=False)
+    with pytest.raises(ValueError, match="momenta"):
+        io.restore_engine(_Eng(), [st], None)
+    with pytest.raises(ValueError, match="one state per system"):
+        io.restore_engine(_Eng(), [st, st], None)

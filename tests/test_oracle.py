@@ -474,3 +474,28 @@ def test_exception_pair_by_hand():
         pos = np.array([[0.0, 0, 0], [0.0, r, 0]])
         e = O.exception_energy(pos, pairs, np.array([sig]), np.array([eps]), np.array([qq]))
         assert e == pytest.approx(lj + 332.0637 * qq / r, rel=1e-9, abs=1e-12)
+
+
+def test_sd_backtrack_in_a_bowl():
+    """E = k/2 |x|^2 with every |F| < max_disp / dt_max, so dt = dt_max and the
+    candidates are x (1 - f dt k). With dt k = 4 their energies scale as
+    (1 - 4 f)^2 = 9, 1, 0, 0.36 for f = 1, 0.5, 0.25, 0.1: the search must
+    take f = 0.25, which lands on the minimum exactly."""
+    k = 1000.0
+    efn = lambda x: 0.5 * k * float(np.sum(x * x))
+    ffn = lambda x: -k * x
+    x0 = np.array([[0.01, -0.005, 0.002], [0.0, 0.003, -0.004]])
+    x1, frac, es, e0 = O.sd_backtrack_step(efn, ffn, x0)
+    assert frac == 0.25
+    assert np.allclose(x1, 0.0, atol=1e-15)
+    assert es == pytest.approx([9 * e0, e0, 0.0, 0.36 * e0], rel=1e-12, abs=1e-18)
+    # dt k = 1.5: (1 - 1.5 f)^2 = 0.25, 0.0625, 0.390625, 0.7225 -> f = 0.5
+    x1, frac, _, _ = O.sd_backtrack_step(efn, ffn, x0, dt_max=1.5 / k)
+    assert frac == 0.5 and np.allclose(x1, 0.25 * x0)
+    # a force-limited step: max|F| = k |x|max, dt = 0.1 / max|F|
+    xb = np.array([[1.0, 0.0, 0.0], [0.0, 0.5, 0.0]])
+    x1, frac, _, _ = O.sd_backtrack_step(efn, ffn, xb)
+    assert frac == 1.0 and np.allclose(x1, xb * (1.0 - 0.1 / 1.0))
+    # at the minimum nothing is lower than E(r): r stays
+    x1, frac, _, _ = O.sd_backtrack_step(efn, ffn, np.zeros((2, 3)))
+    assert frac is None and np.array_equal(x1, np.zeros((2, 3)))

@@ -9,7 +9,7 @@ from prolix_amd import tune
 
 
 def test_defaults_match_the_kernel_source():
-    src = (tune.CSRC / "prx_pair.hip").read_text()
+    src = (tune.CSRC / "prx_pair_dev.h").read_text()
     for k, v in tune.DEFAULTS.items():
         m = re.search(rf"#ifndef {k}\n#define {k} (-?\d+)\n#endif", src)
         assert m, k
