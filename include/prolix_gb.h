@@ -158,6 +158,24 @@ int prx_engine_layout(const prx_engine* eng, int32_t* atom_offsets, int32_t capa
 int prx_energy_forces(prx_engine* eng, const float* positions, float* forces, double* energies,
                       float* born_radii, void* stream);
 
+/* Allocation-free variant of prx_energy_forces for one system given by its
+ * description, with no engine: every device buffer it needs (parameters,
+ * topology, work list, partial sums) is carved from a static device array of
+ * 32 MiB that each GPU holds from library load until the process ends; the
+ * call makes no hipMalloc / hipFree and creates no stream. No counterpart in
+ * prolix: it is for callers that cannot allocate device memory (code inside a
+ * serving loop or harness that forbids allocation, or a stateless custom-call
+ * handler that keeps no engine between calls).
+ * Limits: n_atoms <= 8192, and the system's buffers must fit the 32 MiB
+ * (PRX_ERR_CAPACITY otherwise; an 8,192-atom protein needs about 24 MiB).
+ * positions / forces / energies / born_radii as for prx_energy_forces (one
+ * system: energies is one row). The topology is read from host memory and
+ * uploaded with synchronous copies; the call returns after its results are
+ * written. All calls on one GPU share the array: calls that overlap in time
+ * on one GPU (other host threads, other streams) race and must not be made. */
+int prx_energy_forces_noalloc(const prx_system_desc* desc, const float* positions, float* forces,
+                              double* energies, float* born_radii, void* stream);
+
 /* ---- integrator: nvt_langevin init_fn / apply_fn ------------------------
  * prx_integrator_init = init_fn(key, R, mass) (settle.py:1676-1697):
  * positions (device, packed) become the state; momenta (device, packed) or

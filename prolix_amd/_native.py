@@ -70,6 +70,8 @@ _SIGS = {
     "prx_engine_layout": (ctypes.c_int, [ctypes.c_void_p, c_i32p, ctypes.c_int32]),
     "prx_energy_forces": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                          ctypes.c_void_p, ctypes.c_void_p]),
+    "prx_energy_forces_noalloc": (ctypes.c_int, [ctypes.POINTER(SystemDesc), ctypes.c_void_p, ctypes.c_void_p,
+                                                 ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "prx_integrator_init": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                            ctypes.POINTER(LangevinParams), ctypes.c_void_p]),
     "prx_integrator_steps": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]),

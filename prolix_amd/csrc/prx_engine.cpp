@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -233,6 +234,37 @@ void validate(const prx_system_desc& d) {
   }
 }
 
+}  // namespace
+
+// ---- allocation-free evaluation (prx_energy_forces_noalloc) ------------------
+// Every buffer of the one-shot evaluation is carved from this file-scope
+// device array. Each GPU the library runs on holds its own copy (loaded with
+// the code object) until the process ends. Host code never passes the
+// array's name to a kernel or copy: the device address comes from
+// hipGetSymbolAddress, once per device, and is kept in g_scratch_ptr.
+constexpr size_t kScratchBytes = (size_t)32 << 20;
+constexpr int kScratchMaxAtoms = 8192;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The scratch array's device address on the current device.
+char* scratch_ptr() {
+  int dev = 0;
+  HIPCHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) throw std::invalid_argument("device index out of range for the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    HIPCHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw HipError("hipGetSymbolAddress returned no address for the scratch array");
+    g_scratch_ptr[dev] = p;
+  }
+  return static_cast<char*>(g_scratch_ptr[dev]);
+}
 }  // namespace
 
 struct prx_engine {
@@ -1080,6 +1112,7 @@ int prx_system_create(const prx_system_desc* descs, int32_t n_systems, prx_engin
     eng->a_tp = RangeAlloc(eng->cap_tp);
     eng->a_ent = RangeAlloc(eng->cap_ent);
     try {
+      scratch_ptr();  // set-up of the allocation-free entry point's scratch array on this device
       eng->alloc_device();
       for (int k = 0; k < n_systems; ++k) eng->admit(descs[k]);
       eng->rebuild();
@@ -1121,6 +1154,7 @@ int prx_pool_create(const prx_pool_limits* lim, prx_engine** out) {
     eng->a_coeff = RangeAlloc(eng->cap_coeff);
     eng->a_xt = RangeAlloc(eng->cap_xt);
     try {
+      scratch_ptr();  // set-up of the allocation-free entry point's scratch array on this device
       eng->alloc_device();
     } catch (...) {
       eng->release_device();
@@ -1163,6 +1197,70 @@ int prx_energy_forces(prx_engine* eng, const float* positions, float* forces, do
     if (forces) prx::launch_copy_out(ev, nullptr, nullptr, forces, eng->d_map, n, s);
     if (born_radii) prx::launch_born_out(ev, born_radii, eng->d_map, n, s);
     HIPCHECK(hipGetLastError());
+  });
+}
+
+int prx_energy_forces_noalloc(const prx_system_desc* desc, const float* positions, float* forces,
+                              double* energies, float* born_radii, void* stream) {
+  return guard([&] {
+    need(desc && positions, "prx_energy_forces_noalloc: bad arguments");
+    const prx_system_desc& d = *desc;
+    validate(d);
+    need(d.n_atoms <= kScratchMaxAtoms, "prx_energy_forces_noalloc: at most 8192 atoms");
+    hipStream_t s = (hipStream_t)stream;
+    char* base = scratch_ptr();
+    // The previous call on this stream has finished (each call returns after
+    // its results are written); its scratch contents are overwritten below.
+    HIPCHECK(hipStreamSynchronize(s));
+    prx_engine eng;
+    eng.arena = base;
+    eng.arena_cap = kScratchBytes;
+    eng.fork_min_atoms = INT64_MAX;  // one stream: the caller's
+    prx::ExclPlan plan;
+    prx::build_exclusions(d, plan);
+    eng.cap_atoms = d.n_atoms;
+    eng.cap_bond = d.n_bonds + d.n_ub;
+    eng.cap_ang = d.n_angles;
+    eng.cap_tors = count_torsion_entries(d);
+    eng.cap_cmap = d.n_cmap;
+    eng.cap_exc = d.n_exceptions;
+    eng.cap_coeff = d.n_cmap ? (int64_t)d.n_cmap_maps * d.cmap_grid * d.cmap_grid * 16 : 0;
+    eng.cap_xt = (int64_t)plan.tiles.size();
+    eng.cap_rl = std::max<int64_t>(count_records(d), 1);
+    eng.cap_slots = 1;
+    int64_t n_tp = 0, n_ent = 0;
+    const int64_t n_items = plan_items(d, plan, &n_tp, &n_ent);
+    eng.cap_tp = n_tp;
+    eng.cap_ent = n_ent;
+    eng.a_tp = RangeAlloc(eng.cap_tp);
+    eng.a_ent = RangeAlloc(eng.cap_ent);
+    eng.a_atoms = RangeAlloc(eng.cap_atoms);
+    eng.a_bond = RangeAlloc(eng.cap_bond);
+    eng.a_ang = RangeAlloc(eng.cap_ang);
+    eng.a_tors = RangeAlloc(eng.cap_tors);
+    eng.a_cmap = RangeAlloc(eng.cap_cmap);
+    eng.a_exc = RangeAlloc(eng.cap_exc);
+    eng.a_coeff = RangeAlloc(eng.cap_coeff);
+    eng.a_xt = RangeAlloc(eng.cap_xt);
+    eng.a_rl = RangeAlloc(eng.cap_rl);
+    const int64_t nT = (d.n_atoms + prx::kTile - 1) / prx::kTile;
+    try {
+      eng.alloc_device(n_items, nT);  // CapacityError if the system does not fit
+      eng.admit(d);
+      eng.rebuild();
+      const int n = eng.total_atoms();
+      const prx::DeviceView ev = eng.eval_view();
+      prx::launch_copy_in_positions(ev, positions, eng.d_map, n, s);
+      eng.evaluate(ev, energies != nullptr, s);
+      if (energies) prx::launch_energy_reduce(ev, eng.d_ranges, 1, energies, s);
+      if (forces) prx::launch_copy_out(ev, nullptr, nullptr, forces, eng.d_map, n, s);
+      if (born_radii) prx::launch_born_out(ev, born_radii, eng.d_map, n, s);
+      HIPCHECK(hipGetLastError());
+      HIPCHECK(hipStreamSynchronize(s));
+    } catch (...) {
+      hipStreamSynchronize(s);
+      throw;
+    }
   });
 }
 
