@@ -14,7 +14,12 @@ the plain one-system-per-engine path on the same mix, and a CPU baseline
 - "serving": a finite seeded Poisson trace (50 requests/s, N log-uniform in
   [200, 8000], n_steps uniform in [1000, 50000], T U(280, 350) K, own Philox
   seed each) replayed through the continuous-batching loop: p50/p99
-  completion latency, aggregate ns/day, offered vs achieved pair-steps/s;
+  completion latency, aggregate ns/day, offered vs achieved pair-steps/s
+  (that trace offers far more than the device achieves, so its latencies are
+  queueing); "serving.sustained": the same sizes at 8 requests/s with n_steps
+  U[1000, 5000], a load the device sustains, p50/p99 there;
+  "serving.mix": the batched mix served (512 requests, 256 live at most),
+  its ns/day against the batched figure;
 - "large_cutoff": one 20,000-atom globule with a 20 A cutoff and the
   neighbour-list GB form, ms/step and interactions/s within the cutoff.
 
@@ -107,9 +112,11 @@ def pass_timings(eng, n_steps):
 
 
 def single_system(sysd, gpu_steps, label):
-    """One system alone on the plain GPU path (its own engine, graph-captured
-    steps). Returns the result and the relaxed start positions, from which the
-    CPU baseline leg times the C++ restatement on the same system."""
+    """One system alone on the plain GPU path (its own engine, the default
+    step path: the fused kernel for small systems, graph-captured steps
+    otherwise), and each step path on request. Returns the result and the
+    relaxed start positions, from which the CPU baseline leg times the C++
+    restatement on the same system."""
     import torch
 
     from prolix_amd.engine import GBEngine, Langevin
@@ -118,16 +125,23 @@ def single_system(sysd, gpu_steps, label):
     n = len(sysd["charges"])
     eng = GBEngine(sysd, cutoff=0.0)
     x = minimize(eng, sysd["positions"], steps=100)
-    eng.init(x, Langevin(DT_FS, 300.0, 1.0, seed=3, safety_caps=True))
-    eng.steps(100)
-    torch.cuda.synchronize()
-    t = time.perf_counter()
-    eng.steps(gpu_steps)
-    torch.cuda.synchronize()
-    g = (time.perf_counter() - t) / gpu_steps
+
+    def timed(mode, steps):
+        eng.set_step_mode(mode)
+        eng.init(x, Langevin(DT_FS, 300.0, 1.0, seed=3, safety_caps=True))
+        eng.steps(100)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        eng.steps(steps)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / steps
+
+    g = timed("auto", gpu_steps)  # the engine's default path
+    modes = {m: timed(m, min(gpu_steps, 2000)) * 1e3 for m in ("graph", "persistent", "fused")}
     eng.close()
     out = {"workload": label, "atoms": n, "gpu_steps": gpu_steps, "gpu_ms_per_step": g * 1e3,
-           "gpu_ns_per_day": ns_per_day(1.0 / g), "gpu_pair_interactions_per_s": n * (n - 1) / 2 / g}
+           "gpu_ns_per_day": ns_per_day(1.0 / g), "gpu_pair_interactions_per_s": n * (n - 1) / 2 / g,
+           "ms_per_step_by_step_mode": modes}
     return out, np.ascontiguousarray(x.cpu().numpy(), np.float32)
 
 
@@ -164,6 +178,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=32)
+    ap.add_argument("--sustained-requests", type=int, default=120)
+    ap.add_argument("--sustained-rate", type=float, default=8.0, help="requests/s of the sustained-load trace")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--no-single", action="store_true",
@@ -186,11 +202,14 @@ def main():
     n_mix = len(jobs)
     if not a.no_serve:
         jobs += synth.mix_jobs(a.serve_requests, 200, 8000, seed=5)
+        jobs += synth.mix_jobs(a.sustained_requests, 200, 8000, seed=8)
     if not a.no_large:
         jobs.append((20000, 7))
     built = synth.build_many(jobs)
     prots, temps = built[:n_mix], synth.mix_temperatures(a.n_systems, seed=2)
     serve_prots = built[n_mix:n_mix + (0 if a.no_serve else a.serve_requests)]
+    sustained_prots = built[n_mix + len(serve_prots):n_mix + len(serve_prots)
+                            + (0 if a.no_serve else a.sustained_requests)]
     large = None if a.no_large else built[-1]
     singles = []
     if not a.no_single:
@@ -335,7 +354,21 @@ def main():
         reqs = poisson_trace([p.system_dict() for p in serve_prots], rate_per_s=50.0, steps_range=(1000, 50000),
                              seed=6)
         t1 = time.time()
-        _, st = serve(reqs, hbm_budget_bytes=200e9, chunk_steps=200)
+        results, st = serve(reqs, hbm_budget_bytes=200e9, chunk_steps=200)
+        # a seeded 5% sample (at least 2) of the served requests: the device's
+        # energies and forces at their final positions, checked against the
+        # float64 restatement in the CPU leg
+        rng_s = np.random.default_rng(17)
+        pick = sorted(rng_s.choice(len(results), size=max(2, len(results) // 20), replace=False).tolist())
+        served_sample = []
+        for k in pick:
+            r = results[k]
+            sysd = reqs[r.rid].system
+            e1 = GBEngine(sysd, cutoff=0.0)
+            e_dev, f_dev, _ = e1.energy_forces(r.positions, energies=True)
+            served_sample.append((r.rid, sysd, np.asarray(r.positions, np.float32), e_dev.cpu().numpy()[0],
+                                  f_dev.cpu().numpy()))
+            e1.close()
         offered_rate = 50.0 * st["offered_pair_steps"] / len(reqs)
         serving = {"requests": st["requests"], "rate_per_s": 50.0, "latency_p50_s": st["latency_p50_s"],
                    "latency_p99_s": st["latency_p99_s"], "aggregate_ns_per_day": st["aggregate_ns_per_day"],
@@ -343,11 +376,59 @@ def main():
                    "offered_pair_steps_per_s": offered_rate,
                    "trace": f"{len(reqs)} requests, Poisson 50/s, N log-uniform [200, 8000], "
                             "n_steps U[1000, 50000], T U(280, 350) K, HBM budget 200 GB",
-                   "accuracy": "energy / force error vs the float64 oracle on a seeded 5% sample of a served trace "
-                               "(at least 2 requests): tests/test_gpu_serving.py::test_serving_loop_small_trace, "
-                               "energy rel <= 1e-6, force RMS rel <= 1e-5"}
+                   "failed_requests": st["failed"],
+                   "accuracy": None}
+        serving["host_time_s"] = st["host_time_s"]
         log(f"serving: {len(reqs)} requests in {time.time() - t1:.1f}s, p50 {st['latency_p50_s']:.2f}s "
-            f"p99 {st['latency_p99_s']:.2f}s")
+            f"p99 {st['latency_p99_s']:.2f}s, host {json.dumps({k: round(v, 2) for k, v in st['host_time_s'].items()})}")
+
+        # the same size distribution at a load the device sustains: fewer
+        # steps per request and a lower arrival rate, chunks of about 50 ms
+        reqs = poisson_trace([p.system_dict() for p in sustained_prots], rate_per_s=a.sustained_rate,
+                             steps_range=(1000, 5000), seed=9)
+        span = reqs[-1].arrival
+        results, st = serve(reqs, hbm_budget_bytes=200e9, chunk_steps=1000, chunk_s=0.05)
+        offered = st["offered_pair_steps"] / max(span, 1e-9)
+        sustained = {"requests": st["requests"], "rate_per_s": a.sustained_rate,
+                     "trace": f"{len(reqs)} requests, Poisson {a.sustained_rate:g}/s, N log-uniform [200, 8000], "
+                              "n_steps U[1000, 5000], T U(280, 350) K; chunks sized to about 50 ms",
+                     "latency_p50_s": st["latency_p50_s"], "latency_p99_s": st["latency_p99_s"],
+                     "offered_pair_steps_per_s": offered, "achieved_pair_steps_per_s": st["pair_interactions_per_s"],
+                     "aggregate_ns_per_day": st["aggregate_ns_per_day"], "wall_s": st["wall_s"],
+                     "chunks": st["chunks"], "failed_requests": st["failed"], "host_time_s": st["host_time_s"]}
+        log(f"sustained serving: offered {offered:.3e} pair-steps/s, achieved {st['pair_interactions_per_s']:.3e}, "
+            f"p50 {st['latency_p50_s']:.3f}s p99 {st['latency_p99_s']:.3f}s, "
+            f"host {json.dumps({k: round(v, 2) for k, v in st['host_time_s'].items()})}")
+
+        # the batched mix itself served: each system twice (512 requests, all
+        # queued at the start, n_steps U[1000, 3000]) with at most 256 live,
+        # so finished systems are replaced while the others run
+        rng_m = np.random.default_rng(10)
+        xs_mix = np.split(x.cpu().numpy(), np.cumsum(sizes)[:-1])
+        order = rng_m.permutation(2 * len(systems)) % len(systems)
+        from prolix_amd.serving import Request
+
+        mreqs = []
+        for q, k in enumerate(order):
+            sd = dict(systems[k])
+            sd["positions"] = xs_mix[k]
+            mreqs.append(Request(sd, int(rng_m.integers(1000, 3001)), float(temps[k]), 5000 + q, 0.0, DT_FS, 1.0, q))
+        results, st = serve(mreqs, hbm_budget_bytes=200e9, max_systems=len(systems), chunk_steps=200)
+        ul = st["until_last_admission"]
+        mix_served = {"requests": st["requests"], "max_live": len(systems),
+                      "trace": f"the {len(systems)}-system mix at its relaxed positions, each system twice in a "
+                               "seeded order, all queued at t = 0, n_steps U[1000, 3000], chunks of 200 steps",
+                      "aggregate_ns_per_day": st["aggregate_ns_per_day"],
+                      "aggregate_ns_per_day_until_last_admission": ul["aggregate_ns_per_day"],
+                      "batched_ns_per_day": value,
+                      "served_over_batched": ul["aggregate_ns_per_day"] / value,
+                      "wall_s": st["wall_s"], "until_last_admission_s": ul["wall_s"], "chunks": st["chunks"],
+                      "failed_requests": st["failed"], "host_time_s": st["host_time_s"]}
+        log(f"served mix: {ul['aggregate_ns_per_day']:.1f} ns/day until the last admission "
+            f"({mix_served['served_over_batched']:.3f} of batched), {st['aggregate_ns_per_day']:.1f} over the whole "
+            f"run, host {json.dumps({k: round(v, 2) for k, v in st['host_time_s'].items()})}")
+        serving["sustained"] = sustained
+        serving["mix"] = mix_served
 
     # 8. one system alone (the 138-atom peptide, the 2,500-atom globule), GPU and CPU
     single_out, single_x = [], []
@@ -355,7 +436,8 @@ def main():
         r, xs = single_system(sd, gsteps, label)
         single_out.append(r)
         single_x.append(xs)
-        log(f"single system {r['atoms']} atoms: {r['gpu_ms_per_step']:.4f} ms/step GPU")
+        log(f"single system {r['atoms']} atoms: {r['gpu_ms_per_step']:.4f} ms/step GPU (default path); by mode "
+            f"{json.dumps({k: round(v, 4) for k, v in r['ms_per_step_by_step_mode'].items()})}")
 
     # 9. CPU baseline (bounded sample)
     cpu = None
@@ -363,6 +445,27 @@ def main():
         t0 = time.time()
         cpu = cpu_baseline(systems, sizes.astype(np.float64))
         log(f"cpu baseline {cpu['value']:.3f} ns/day ({time.time() - t0:.0f}s)")
+        # the served sample against the float64 restatement (the checker)
+        if serving is not None:
+            from oracle import cpu as C
+
+            def f32(d):  # the device's float32 parameters, held in float64 for the checker
+                return {k: (np.asarray(v, np.float32).astype(np.float64)
+                            if isinstance(v, np.ndarray) and v.dtype.kind == "f" else v) for k, v in d.items()}
+
+            errs = []
+            for rid, sysd, xs, e_dev, f_dev in served_sample:
+                eo, fo = C.CpuSystem(f32(sysd)).energy_forces(xs.astype(np.float64), np.float64)
+                errs.append({"request": rid, "atoms": len(xs), "energy_rel": float(abs(e_dev[-1] - eo[11]) / abs(eo[11])),
+                             "force_rms_rel": float(np.sqrt(np.mean((f_dev - fo) ** 2) / np.mean(fo**2)))})
+            serving["accuracy"] = {
+                "sample": f"{len(errs)} of {serving['requests']} served requests (seeded 5%, at least 2), at their "
+                          "final positions, device vs the float64 C++ restatement",
+                "energy_rel_max": max(e["energy_rel"] for e in errs),
+                "force_rms_rel_max": max(e["force_rms_rel"] for e in errs),
+                "per_request": errs}
+            log(f"served sample vs float64: energy rel <= {serving['accuracy']['energy_rel_max']:.2e}, "
+                f"force rms rel <= {serving['accuracy']['force_rms_rel_max']:.2e}")
         # the same restatement beside each one-system GPU leg, from the same start
         for r, (label, sd, gsteps, csteps), xs in zip(single_out, singles, single_x):
             r.update(cpu_single(sd, xs, csteps, gsteps))
@@ -390,6 +493,8 @@ def main():
         "pair_interactions_per_s": pairs_per_step * sps,
         "latency_p50_s": serving["latency_p50_s"] if serving else None,
         "latency_p99_s": serving["latency_p99_s"] if serving else None,
+        "latency_sustained_p50_s": serving["sustained"]["latency_p50_s"] if serving else None,
+        "latency_sustained_p99_s": serving["sustained"]["latency_p99_s"] if serving else None,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "plain_path": plain,

@@ -228,15 +228,20 @@ int prx_integrator_health(prx_engine* eng, int64_t* nonfinite, void* stream);
  * Replaces make_neighbor_list_fn's jax_md list (physics/neighbor_list.py:155-172,
  * dr_threshold) and its rebuild rule. */
 int prx_engine_set_nl_skin(prx_engine* eng, double skin);
-/* How prx_integrator_steps runs the steps: 1 = captured HIP graphs of the
- * per-pass kernels; 2 = one persistent cooperative kernel per 1,000 steps
- * whose passes are separated by grid-wide barriers (the same per-item and
- * per-tile code, so the trajectory is the same bit for bit; it removes the
- * per-pass launch latency that bounds small engines, and the call returns
- * after the steps are done); 0 = auto (default): persistent for engines of
- * at most 16,384 work items, graphs above. PRX_ERR_STATE for 2 where
- * cooperative launches are unavailable. No counterpart in prolix (its loop
- * is jax.lax.fori_loop, physics/simulate.py:253-254). */
+/* How prx_integrator_steps runs the steps. Every path runs the same
+ * per-item and per-tile device code on the same work lists, so the
+ * trajectory is the same bit for bit:
+ * 1 = captured HIP graphs of the per-pass kernels;
+ * 2 = one persistent cooperative kernel per 1,000 steps whose passes are
+ *     separated by grid-wide barriers (the call returns after the steps);
+ * 3 = fused: one kernel per 1,000 steps in which one workgroup runs all the
+ *     passes of one system with workgroup barriers only, systems side by
+ *     side; PRX_ERR_STATE if a system uses a neighbour list;
+ * 0 = auto (default): graphs, the fastest of the three on MI355X for every
+ *     size measured (138 to 2,500 atoms).
+ * PRX_ERR_STATE for 2 where cooperative launches are unavailable. No
+ * counterpart in prolix (its loop is jax.lax.fori_loop,
+ * physics/simulate.py:253-254). */
 int prx_engine_set_step_mode(prx_engine* eng, int32_t mode);
 /* Run the bonded kernel on a second stream beside the pair passes (a
  * parallel branch of the step graph) when the engine holds at least
@@ -280,10 +285,19 @@ typedef struct prx_pool_limits {
 
 int prx_pool_create(const prx_pool_limits* limits, prx_engine** out);
 /* Admit one system with its start positions (host, (n,3) float32) and
- * integrator parameters; momenta are drawn from its Philox stream. The new
- * system joins the next prx_integrator_steps call. slot receives its id. */
+ * integrator parameters. The new system joins the next prx_integrator_steps
+ * call: its momenta are drawn from its Philox stream and the forces computed
+ * then (or at the next read of the pool), once for every system admitted
+ * since the last step call. slot receives its id. */
 int prx_pool_admit(prx_engine* pool, const prx_system_desc* desc, const float* positions_host,
                    const prx_langevin_params* params, int32_t* slot, void* stream);
+/* Stop one system after limit BAOAB steps in all (0: no limit, the default
+ * at admission). A system at its limit keeps its positions, momenta and step
+ * count while the others step on, so a step call may run past the end of one
+ * request without changing its result. Synchronises the streams. Serving
+ * only; prolix runs every system of a bucket for the same step count
+ * (batched_simulate.py:1125-1422). */
+int prx_pool_set_step_limit(prx_engine* pool, int32_t slot, int64_t limit, void* stream);
 /* Remove a system; its arena ranges are reused by later admissions. */
 int prx_pool_release(prx_engine* pool, int32_t slot, void* stream);
 /* Copy one system's positions / momenta / forces to host ((n,3) float32 each,
@@ -294,6 +308,18 @@ int prx_pool_read(prx_engine* pool, int32_t slot, float* positions, float* momen
 int prx_pool_slots(const prx_engine* pool, int32_t* slots, int32_t capacity, int32_t* n_live);
 /* Energies of one live slot at its current positions (host out, PRX_N_ENERGY doubles). */
 int prx_pool_energy(prx_engine* pool, int32_t slot, double* energies_host, void* stream);
+
+/* ---- neighbour-list object -------------------------------------------------
+ * The list of make_neighbor_list_fn (physics/neighbor_list.py:155-172, a
+ * jax_md Dense list with mask_self): for each atom i of the (n, 3) float32
+ * device positions, the atoms j != i closer than radius (cutoff + the
+ * list's skin) in increasing j, written to the device (n, capacity) int32
+ * idx with n after the last entry; counts (device, n int32) receives the
+ * true numbers, so counts[i] > capacity flags an overflow. workspace: device
+ * float array of 8 * ceil(n / 64). Stream-ordered. The engine's own pair
+ * work does not use this list (gb_cut systems build their tile lists). */
+int prx_neighbor_list_dense(const float* positions, int32_t n_atoms, double radius, int32_t capacity,
+                            int32_t* idx, int32_t* counts, float* workspace, void* stream);
 
 /* ---- host-side planning (no device work; usable without a GPU) --------- */
 

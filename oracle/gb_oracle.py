@@ -30,6 +30,9 @@ under prolix's ``src/prolix/``):
 - ``fire_minimize``            simulate.py:615-748 (jax_md fire_descent, capped apply)
 - ``capped_descent``           batched_simulate.py:746-787 (per-atom capped SD)
 - ``sd_backtrack_step``        physics/simulate.py:141-196 (SD + 4-point backtracking)
+- ``fire_stage_best``          batched_simulate.py:813-874 (restrained FIRE sub-stage)
+- ``lbfgs_minimize``           batched_simulate.py:886-935 (the L-BFGS polish; jaxopt's own
+                               line search is not restated, see the function)
 
 Forces are this module's own analytic derivatives of those energies. Tests pin
 them with central finite differences of the energies in float64, Newton's
@@ -57,7 +60,11 @@ value fixed outside this module:
 - ``forces``: central finite differences, Newton's third law, rigid motion;
 - the minimisers: a harmonic bowl's known path, cap bounds, a restraint balance;
   the backtracking search: a bowl whose candidate energies are known in
-  closed form, so the selection order and the strict comparisons are fixed.
+  closed form, so the selection order and the strict comparisons are fixed;
+  the FIRE sub-stage: a bowl (steps capped at 0.1 A, the best snapshot's RMS
+  force never above any visited); L-BFGS: the exact minimum of an
+  anisotropic quadratic and of the Rosenbrock function, with energies that
+  never rise.
 """
 
 from __future__ import annotations
@@ -849,3 +856,110 @@ def sd_backtrack_step(energy_fn, force_fn, pos, dt_max=4e-3, max_displacement=0.
     if es[best] < e0:
         return cands[best], SD_FRACTIONS[best], es, e0
     return x, None, es, e0
+
+
+# ----------------------------------------------------------------------------
+# The batched minimiser's restrained FIRE sub-stage and L-BFGS polish
+# (batched_simulate.py:801-935), float64, one system
+# ----------------------------------------------------------------------------
+
+def fire_stage_best(force_fn, pos, mass, steps, dt_start=1e-4, dt_max=1e-3, disp_cap=0.1, n_min=5, f_inc=1.1,
+                    f_dec=0.5, alpha_start=0.1, f_alpha=0.99):
+    """One FIRE sub-stage (batched_simulate.py:813-874): non-finite force
+    components -> 0 (no magnitude cap), the per-step displacement capped at
+    disp_cap with |dr| = sqrt(dr.dr + 1e-30) and the momentum scaled alike;
+    the positions with the lowest RMS force seen before a step are returned
+    with that RMS force."""
+    x = np.asarray(pos, np.float64).copy()
+    m = np.asarray(mass, np.float64).reshape(-1, 1)
+    p = np.zeros_like(x)
+    guard = lambda f: np.where(np.isfinite(f), f, 0.0)
+    f = guard(force_fn(x))
+    dt, alpha, n_pos = float(dt_start), float(alpha_start), 0
+    best_x, best_rms = x.copy(), 1e10
+    for _ in range(int(steps)):
+        rms = math.sqrt(float(np.sum(f * f)) / len(x))
+        if rms < best_rms:
+            best_x, best_rms = x.copy(), rms
+        x0 = x
+        p = p + 0.5 * dt * f
+        x = x0 + dt * p / m
+        f = guard(force_fn(x))
+        p = p + 0.5 * dt * f
+        f_norm = math.sqrt(float(np.sum(f * f)) + 1e-6)
+        p_norm = math.sqrt(float(np.sum(p * p)))
+        fdotp = float(np.sum(f * p))
+        p = p + alpha * (f * p_norm / f_norm - p)
+        n_pos = n_pos + 1 if fdotp >= 0 else 0
+        if fdotp > 0 and n_pos > n_min:
+            dt = min(dt * f_inc, dt_max)
+            alpha = alpha * f_alpha
+        if fdotp < 0:
+            dt = dt * f_dec
+            alpha = alpha_start
+            p = np.zeros_like(p)
+        dr = x - x0
+        sc = np.minimum(1.0, disp_cap / np.sqrt(np.sum(dr * dr, axis=1, keepdims=True) + 1e-30))
+        x = x0 + dr * sc
+        p = p * sc
+    return best_x, best_rms
+
+
+def lbfgs_minimize(energy_fn, grad_fn, pos, steps, history=10, max_stepsize=0.1, tol=1e-6, c1=1e-4, shrink=0.5,
+                   max_trials=12):
+    """L-BFGS with a backtracking Armijo search, the method the device runs for
+    the batched minimiser's last stage (batched_simulate.py:886-935 runs
+    jaxopt.LBFGS, history 10, max_stepsize 0.1): two-loop recursion (Nocedal &
+    Wright, Alg. 7.4) with H0 = (s.y / y.y) I over the last ``history`` pairs
+    with s.y > 0; trial steps t = max_stepsize * shrink^k; a failed search or a
+    gradient norm <= tol leaves x unchanged for that iteration; non-finite
+    energies count as 1e10 and non-finite gradient components as 0. Returns
+    (positions, energies per iteration)."""
+    fe = lambda y: (lambda e: e if np.isfinite(e) else 1e10)(float(energy_fn(y)))
+    fg = lambda y: (lambda g: np.where(np.isfinite(g), g, 0.0))(np.asarray(grad_fn(y), np.float64))
+    x = np.asarray(pos, np.float64).copy()
+    e, g = fe(x), fg(x)
+    S, Y = [], []
+    trace = [e]
+    for _ in range(int(steps)):
+        if math.sqrt(float(np.sum(g * g))) <= tol:
+            break
+        q = g.copy()
+        alphas = []
+        for s_, y_ in reversed(list(zip(S, Y))):
+            a = float(np.sum(s_ * q)) / float(np.sum(s_ * y_))
+            alphas.append(a)
+            q = q - a * y_
+        gamma = 1.0
+        if S:
+            sy, yy = float(np.sum(S[-1] * Y[-1])), float(np.sum(Y[-1] * Y[-1]))
+            if sy > 0 and yy > 0:
+                gamma = sy / yy
+        r = gamma * q
+        for (s_, y_), a in zip(zip(S, Y), reversed(alphas)):
+            b = float(np.sum(y_ * r)) / float(np.sum(s_ * y_))
+            r = r + (a - b) * s_
+        d = -r
+        gd = float(np.sum(g * d))
+        if not gd < 0:
+            d, gd = -g, -float(np.sum(g * g))
+        t, step, e_new = float(max_stepsize), 0.0, e
+        for _k in range(int(max_trials)):
+            et = fe(x + t * d)
+            if et <= e + c1 * t * gd:
+                step, e_new = t, et
+                break
+            t *= shrink
+        if step > 0:
+            x_new = x + step * d
+            g_new = fg(x_new)
+            s_v, y_v = x_new - x, g_new - g
+            if float(np.sum(s_v * y_v)) > 1e-300:
+                S.append(s_v)
+                Y.append(y_v)
+                if len(S) > history:
+                    S.pop(0)
+                    Y.pop(0)
+            x, g, e = x_new, g_new, e_new
+        trace.append(e)
+    return x, np.array(trace)

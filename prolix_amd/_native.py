@@ -92,6 +92,7 @@ _SIGS = {
     "prx_pool_admit": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SystemDesc), c_f32p,
                                       ctypes.POINTER(LangevinParams), c_i32p, ctypes.c_void_p]),
     "prx_pool_release": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]),
+    "prx_pool_set_step_limit": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_void_p]),
     "prx_pool_read": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, c_f32p, c_f32p, c_f32p,
                                      ctypes.POINTER(ctypes.c_int64), ctypes.c_void_p]),
     "prx_pool_slots": (ctypes.c_int, [ctypes.c_void_p, c_i32p, ctypes.c_int32, c_i32p]),
@@ -99,6 +100,8 @@ _SIGS = {
     "prx_profile_passes": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, c_f64p]),
     "prx_profile_steps": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, c_f64p]),
     "prx_engine_work_items": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]),
+    "prx_neighbor_list_dense": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_double, ctypes.c_int32,
+                                               ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "prx_cmap_coefficients": (ctypes.c_int, [c_f64p, ctypes.c_int32, c_f64p]),
     "prx_plan_tiles": (ctypes.c_int, [ctypes.POINTER(SystemDesc), ctypes.POINTER(ctypes.c_int64),
                                       ctypes.POINTER(ctypes.c_int64)]),

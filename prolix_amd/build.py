@@ -19,7 +19,7 @@ CSRC = PKG / "csrc"
 LIBDIR = PKG / "lib"
 LIB = LIBDIR / "libprolix_gb.so"
 
-SOURCES = ["prx_pair.hip", "prx_atom.hip", "prx_bonded.hip", "prx_nl.hip", "prx_persist.hip", "prx_host.cpp",
+SOURCES = ["prx_pair.hip", "prx_atom.hip", "prx_bonded.hip", "prx_nl.hip", "prx_persist.hip", "prx_fused.hip", "prx_host.cpp",
            "prx_engine.cpp"]
 HIP_FLAGS = [
     "-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC",

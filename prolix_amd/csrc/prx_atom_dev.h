@@ -425,13 +425,15 @@ __device__ __forceinline__ float4 finish_atom(const DeviceView& v, const TileCtx
   }
   v.force[c.g] = make_float4(f.x, f.y, f.z, 0.0f);
   if (mode < 0 || !S.active) return v.pos[c.g];
+  const uint32_t prev = v.atom_step[c.g];  // each lane owns its atom's counter
+  if (slot_done(S, prev)) return v.pos[c.g];
   float4 p = v.mom[c.g];  // w = 1/m
   p.x = fmaf(S.half_dt, f.x, p.x);
   p.y = fmaf(S.half_dt, f.y, p.y);
   p.z = fmaf(S.half_dt, f.z, p.z);
-  const uint32_t step = v.atom_step[c.g] + 1u;  // each lane owns its atom's counter
+  const uint32_t step = prev + 1u;
   v.atom_step[c.g] = step;
-  float4 x = mode == 1 ? begin_step(v, S, c.slot, c.g, p, step) : v.pos[c.g];
+  float4 x = mode == 1 && !slot_done(S, step) ? begin_step(v, S, c.slot, c.g, p, step) : v.pos[c.g];
   v.mom[c.g] = p;
   // explosion check: count atom-steps that leave a non-finite position or momentum
   if (!(isfinite(x.x) && isfinite(x.y) && isfinite(x.z) && isfinite(p.x) && isfinite(p.y) && isfinite(p.z)))
@@ -481,9 +483,10 @@ __device__ __forceinline__ void step_begin_atom(const DeviceView& v, int g) {
   const int s = v.slot_of[g];
   if (s < 0) return;
   const SlotConst& S = v.slots[s];
-  if (!S.active) return;
+  const uint32_t step = v.atom_step[g];
+  if (!S.active || slot_done(S, step)) return;
   float4 p = v.mom[g];
-  begin_step(v, S, s, g, p, v.atom_step[g]);
+  begin_step(v, S, s, g, p, step);
   v.mom[g] = p;
 }
 

@@ -75,7 +75,16 @@ struct SlotConst {
   int32_t tp_base;      // global index of the system's first tile pair (I < J) record
   float nl_cut2;        // (cutoff + skin)^2 with a 1e-5 relative margin: list membership
   float skin_half2;     // (skin / 2)^2; 0: the list is rebuilt at every evaluation
+  // 0: no limit; else the system stops after this many BAOAB steps: its
+  // positions, momenta and counter stay as they were after that step while the
+  // rest of the batch goes on, and its force is recomputed at those positions
+  uint32_t step_limit;
 };
+
+// True when a slot with a step limit has taken its steps.
+__host__ __device__ __forceinline__ bool slot_done(const SlotConst& S, uint32_t steps) {
+  return S.step_limit != 0u && steps >= S.step_limit;
+}
 
 // Safety caps of prolix's batched BAOAB step (batched_simulate.py:269-358), in
 // the engine's AKMA units: |p/m| <= 20 A/tau, |dx| <= 0.5 A per step,
@@ -189,6 +198,9 @@ void launch_tile_boxes(const DeviceView& v, hipStream_t s);                // ->
 // force, every list is rebuilt; otherwise those whose atoms moved more than
 // skin / 2 since their last build (every list, when skin = 0).
 void launch_nl_build(const DeviceView& v, const int32_t* slot_ranges, int n_slots, bool force, hipStream_t s);
+// Dense per-atom neighbour lists of raw (n, 3) positions (prx_neighbor_list_dense).
+void launch_dense_list(const float* pos, int n, float radius, int capacity, int32_t* idx, int32_t* counts,
+                       float4* boxes, hipStream_t s);
 void launch_born_pairs(const DeviceView& v, hipStream_t s);                // also -> item_cls
 void launch_born_gather(const DeviceView& v, hipStream_t s);               // -> B, dB/dI
 void launch_force_pairs(const DeviceView& v, bool energy, hipStream_t s);
@@ -217,6 +229,9 @@ void launch_tick(unsigned long long* out, hipStream_t s);
 // its blocks (0 if cooperative launches are unavailable), and one launch of
 // n_steps steps with n_blocks blocks (<= that grid).
 int persist_max_blocks();
+// Fused step kernel (prx_fused.hip): n_steps steps of every listed system,
+// one workgroup per system (slot_ranges rows as for launch_kinetic).
+void launch_fused_steps(const DeviceView& v, const int32_t* slot_ranges, int n_slots, int n_steps, hipStream_t s);
 hipError_t launch_persist_steps(const DeviceView& v, const int32_t* slot_ranges, int n_slots, int n_steps,
                                 bool has_nl, GridBarrier* bar, int n_blocks, hipStream_t s);
 void launch_born_out(const DeviceView& v, float* out, const int32_t* map, int n, hipStream_t s);

@@ -288,12 +288,15 @@ struct prx_engine {
   int32_t* d_nl_colitems = nullptr;
   int64_t cap_colitems = 0;
   bool has_nl = false;            // a live system uses its neighbour list
-  // step path: 0 = auto (the persistent kernel for engines of at most
-  // kPersistAutoItems work items, graphs above), 1 = captured graphs,
-  // 2 = the persistent kernel
-  int32_t step_mode = 1;
+  // step path: 0 = auto (graphs: on MI355X neither kernel below beats them,
+  // measured from 138 to 2,500 atoms), 1 = captured graphs, 2 = the
+  // persistent kernel, 3 = the fused kernel (one workgroup per system)
+  int32_t step_mode = 0;
   prx::GridBarrier* d_bar = nullptr;
   int persist_blocks = -1;        // co-resident grid of the persistent kernel (0: unavailable)
+  // pool slots admitted since the last step call: their momenta are drawn and
+  // every live system's forces computed once for the whole batch (flush_pending)
+  std::vector<int32_t> pending_init;
   int32_t* d_rec_list = nullptr;
   std::vector<Slot> slots;
   prx::DeviceView v{};
@@ -737,6 +740,7 @@ struct prx_engine {
 
   void release(int32_t sid) {
     if (sid < 0 || sid >= (int32_t)slots.size() || !slots[sid].live) throw std::invalid_argument("no such live slot");
+    pending_init.erase(std::remove(pending_init.begin(), pending_init.end(), sid), pending_init.end());
     Slot& s = slots[sid];
     HIPCHECK(hipMemset(v.slot_of + s.a0, 0xFF, s.n * sizeof(int32_t)));
     HIPCHECK(hipMemset(v.atom_rec + s.a0, 0, s.n * sizeof(int2)));
@@ -958,20 +962,40 @@ struct prx_engine {
     return exec;
   }
 
-  // Work items up to which the auto step path runs the persistent kernel.
-  static constexpr int64_t kPersistAutoItems = 16384;
-  // Steps per persistent launch: long enough to amortise the launch, short
+  // Momenta of the pool slots admitted since the last flush, then the forces
+  // of every live system at its current positions (a deterministic
+  // evaluation: the other systems' stored forces are reproduced bit for bit).
+  void flush_pending(hipStream_t s) {
+    if (pending_init.empty()) return;
+    rebuild();
+    // the slot list may still be read by an earlier momentum draw
+    HIPCHECK(hipStreamSynchronize(s));
+    HIPCHECK(hipStreamSynchronize(stream));
+    HIPCHECK(hipMemcpy(d_slot_list, pending_init.data(), pending_init.size() * sizeof(int32_t),
+                       hipMemcpyHostToDevice));
+    prx::launch_init_momenta(v, d_slot_list, (int)pending_init.size(), s);
+    evaluate(false, s);
+    HIPCHECK(hipGetLastError());
+    pending_init.clear();
+  }
+
+  // Steps per persistent or fused launch: long enough to amortise the launch, short
   // enough that one launch stays well under a second on the largest engine.
   static constexpr int kPersistChunk = 1000;
 
   bool use_persistent() {
-    if (step_mode == 1) return false;
+    if (step_mode != 2) return false;
     if (persist_blocks < 0) persist_blocks = prx::persist_max_blocks();
-    if (step_mode == 2) {
-      if (persist_blocks <= 0) throw StateError("the persistent step kernel needs cooperative launches");
-      return true;
-    }
-    return persist_blocks > 0 && v.n_items <= kPersistAutoItems;
+    if (persist_blocks <= 0) throw StateError("the persistent step kernel needs cooperative launches");
+    return true;
+  }
+
+  // The fused kernel: every system in one workgroup of its own. Not for
+  // neighbour-list systems (their list build spans workgroups).
+  bool use_fused() const {
+    if (step_mode != 3) return false;
+    if (has_nl) throw StateError("the fused step kernel does not run neighbour-list systems");
+    return true;
   }
 
   // The persistent kernel: at most one item per wave and one tile per block
@@ -992,45 +1016,41 @@ struct prx_engine {
   }
 
   void steps(int n_steps, hipStream_t s) {
+    flush_pending(s);
     rebuild();
     if (order.empty() || n_steps <= 0) return;
-    if (use_persistent()) {
-      hipEvent_t ev;
-      HIPCHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    // the steps run on the engine stream, ordered after the caller's
+    hipEvent_t ev;
+    HIPCHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    try {
       if (s != stream) {
         HIPCHECK(hipEventRecord(ev, s));
         HIPCHECK(hipStreamWaitEvent(stream, ev, 0));
       }
-      try {
+      if (use_fused()) {
+        for (int left = n_steps; left > 0;) {
+          const int k = std::min(left, kPersistChunk);
+          prx::launch_fused_steps(v, d_ranges, (int)order.size(), k, stream);
+          HIPCHECK(hipGetLastError());
+          left -= k;
+        }
+      } else if (use_persistent()) {
         steps_persistent(n_steps, stream);
-      } catch (...) {
-        hipEventDestroy(ev);
-        throw;
+      } else {
+        const int chunk = std::min(n_steps, 50);
+        for (int left = n_steps; left > 0;) {
+          const int k = std::min(chunk, left);
+          HIPCHECK(hipGraphLaunch(capture(k), stream));
+          left -= k;
+        }
       }
       if (s != stream) {
         HIPCHECK(hipEventRecord(ev, stream));
         HIPCHECK(hipStreamWaitEvent(s, ev, 0));
       }
-      HIPCHECK(hipEventDestroy(ev));
-      return;
-    }
-    // Graphs are captured on the engine stream; order them after the caller's stream.
-    const int chunk = std::min(n_steps, 50);
-    hipEvent_t ev;
-    HIPCHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-    if (s != stream) {
-      HIPCHECK(hipEventRecord(ev, s));
-      HIPCHECK(hipStreamWaitEvent(stream, ev, 0));
-    }
-    int left = n_steps;
-    while (left > 0) {
-      const int k = std::min(chunk, left);
-      HIPCHECK(hipGraphLaunch(capture(k), stream));
-      left -= k;
-    }
-    if (s != stream) {
-      HIPCHECK(hipEventRecord(ev, stream));
-      HIPCHECK(hipStreamWaitEvent(s, ev, 0));
+    } catch (...) {
+      hipEventDestroy(ev);
+      throw;
     }
     HIPCHECK(hipEventDestroy(ev));
   }
@@ -1328,6 +1348,7 @@ int prx_engine_set_masses(prx_engine* eng, const double* masses) {
 int prx_integrator_state(prx_engine* eng, float* positions, float* momenta, float* forces, void* stream) {
   return guard([&] {
     need(eng != nullptr, "prx_integrator_state: bad arguments");
+    eng->flush_pending(eng->pick(stream));
     eng->rebuild();
     prx::launch_copy_out(eng->v, positions, momenta, forces, eng->d_map, eng->total_atoms(), eng->pick(stream));
     HIPCHECK(hipGetLastError());
@@ -1337,6 +1358,7 @@ int prx_integrator_state(prx_engine* eng, float* positions, float* momenta, floa
 int prx_integrator_kinetic(prx_engine* eng, double* kinetic, void* stream) {
   return guard([&] {
     need(eng && kinetic, "prx_integrator_kinetic: bad arguments");
+    eng->flush_pending(eng->pick(stream));
     eng->rebuild();
     prx::launch_kinetic(eng->v, eng->d_ranges, (int)eng->order.size(), kinetic, eng->pick(stream));
     HIPCHECK(hipGetLastError());
@@ -1397,6 +1419,7 @@ int prx_integrator_health(prx_engine* eng, int64_t* nonfinite, void* stream) {
   return guard([&] {
     need(eng && nonfinite, "prx_integrator_health: bad arguments");
     hipStream_t s = eng->pick(stream);
+    eng->flush_pending(s);
     HIPCHECK(hipStreamSynchronize(s));
     HIPCHECK(hipStreamSynchronize(eng->stream));
     std::vector<unsigned int> w((size_t)eng->cap_slots * prx::kWarnStride);
@@ -1424,7 +1447,8 @@ int prx_engine_set_nl_skin(prx_engine* eng, double skin) {
 
 int prx_engine_set_step_mode(prx_engine* eng, int32_t mode) {
   return guard([&] {
-    need(eng && mode >= 0 && mode <= 2, "prx_engine_set_step_mode: mode is 0 (auto), 1 (graphs) or 2 (persistent)");
+    need(eng && mode >= 0 && mode <= 3,
+         "prx_engine_set_step_mode: mode is 0 (auto), 1 (graphs), 2 (persistent) or 3 (fused)");
     if (mode == 2) {
       if (eng->persist_blocks < 0) eng->persist_blocks = prx::persist_max_blocks();
       if (eng->persist_blocks <= 0) throw StateError("the persistent step kernel needs cooperative launches");
@@ -1460,19 +1484,28 @@ int prx_pool_admit(prx_engine* pool, const prx_system_desc* desc, const float* p
       for (int i = 0; i < sl.n; ++i)
         p4[i] = make_float4(positions_host[3 * i], positions_host[3 * i + 1], positions_host[3 * i + 2], 0.f);
       HIPCHECK(hipMemcpy(pool->v.pos + sl.a0, p4.data(), sl.n * sizeof(float4), hipMemcpyHostToDevice));
-      pool->rebuild();
-      HIPCHECK(hipMemcpy(pool->d_slot_list, &sid, sizeof(int32_t), hipMemcpyHostToDevice));
-      prx::launch_init_momenta(pool->v, pool->d_slot_list, 1, s);
-      // Forces for every live system at its current positions (deterministic, so
-      // the others' stored forces are reproduced bit for bit).
-      pool->evaluate(false, s);
-      HIPCHECK(hipGetLastError());
-      HIPCHECK(hipStreamSynchronize(s));
+      // momenta and forces follow at the next step call or read, once for
+      // every system admitted until then
+      pool->pending_init.push_back(sid);
     } catch (...) {
       pool->release(sid);
       throw;
     }
     *slot = sid;
+  });
+}
+
+int prx_pool_set_step_limit(prx_engine* pool, int32_t slot, int64_t limit, void* stream) {
+  return guard([&] {
+    need(pool && pool->pool && slot >= 0 && slot < (int32_t)pool->slots.size() && pool->slots[slot].live,
+         "prx_pool_set_step_limit: no such live slot");
+    need(limit >= 0 && limit < (int64_t)UINT32_MAX, "prx_pool_set_step_limit: 0 <= limit < 2^32 - 1 required");
+    // kernels in flight read the slot table: replace the entry between launches
+    HIPCHECK(hipStreamSynchronize(pool->pick(stream)));
+    HIPCHECK(hipStreamSynchronize(pool->stream));
+    prx::SlotConst& c = pool->slots[slot].c;
+    c.step_limit = (uint32_t)limit;
+    pool->push_const(slot, c);
   });
 }
 
@@ -1491,6 +1524,7 @@ int prx_pool_read(prx_engine* pool, int32_t slot, float* positions, float* momen
     need(pool && slot >= 0 && slot < (int32_t)pool->slots.size() && pool->slots[slot].live,
          "prx_pool_read: no such live slot");
     hipStream_t s = pool->pick(stream);
+    pool->flush_pending(s);
     HIPCHECK(hipStreamSynchronize(s));
     HIPCHECK(hipStreamSynchronize(pool->stream));
     const Slot& sl = pool->slots[slot];
@@ -1526,8 +1560,9 @@ int prx_pool_energy(prx_engine* pool, int32_t slot, double* energies_host, void*
   return guard([&] {
     need(pool && energies_host && slot >= 0 && slot < (int32_t)pool->slots.size() && pool->slots[slot].live,
          "prx_pool_energy: no such live slot");
-    pool->rebuild();
     hipStream_t s = pool->pick(stream);
+    pool->flush_pending(s);
+    pool->rebuild();
     HIPCHECK(hipStreamSynchronize(pool->stream));
     // evaluated in the evaluation buffers at a copy of the positions: the
     // integrator's stored forces stay those of its own force-only passes
@@ -1675,6 +1710,17 @@ int prx_engine_work_items(const prx_engine* eng, int64_t* n_items) {
   return guard([&] {
     need(eng && n_items, "prx_engine_work_items: bad arguments");
     *n_items = eng->v.n_items;
+  });
+}
+
+int prx_neighbor_list_dense(const float* positions, int32_t n_atoms, double radius, int32_t capacity,
+                            int32_t* idx, int32_t* counts, float* workspace, void* stream) {
+  return guard([&] {
+    need(positions && idx && counts && workspace && n_atoms > 0 && capacity > 0 && radius > 0.0,
+         "prx_neighbor_list_dense: bad arguments");
+    prx::launch_dense_list(positions, n_atoms, (float)radius, capacity, idx, counts,
+                           reinterpret_cast<float4*>(workspace), (hipStream_t)stream);
+    HIPCHECK(hipGetLastError());
   });
 }
 

@@ -52,7 +52,83 @@ __global__ __launch_bounds__(kNlThreads) void k_nl_rows(DeviceView v) {
   nl_rows_wave(v, t);
 }
 
+// ---- dense atom lists (the drop-in neighbour-list object) -----------------
+// Per atom i, every atom j != i closer than radius, in increasing j, in the
+// (N, K) layout of a jax_md Dense neighbour list (sentinel N after the last
+// entry); counts[i] = the true number, which may exceed K (overflow).
+
+// Box of tile t of raw (N, 3) positions: (lo, hi) at boxes[2t], [2t + 1].
+__global__ __launch_bounds__(kNlThreads) void k_dense_boxes(const float* pos, int n, float4* boxes) {
+  const int t = wave_index();
+  const int n_tiles = (n + kTile - 1) / kTile;
+  if (t >= n_tiles) return;
+  const int lane = threadIdx.x & 63;
+  const int i = min(t * kTile + lane, n - 1);
+  float lx = pos[3 * i], ly = pos[3 * i + 1], lz = pos[3 * i + 2];
+  float hx = lx, hy = ly, hz = lz;
+  for (int off = 32; off > 0; off >>= 1) {
+    lx = fminf(lx, __shfl_xor(lx, off));
+    ly = fminf(ly, __shfl_xor(ly, off));
+    lz = fminf(lz, __shfl_xor(lz, off));
+    hx = fmaxf(hx, __shfl_xor(hx, off));
+    hy = fmaxf(hy, __shfl_xor(hy, off));
+    hz = fmaxf(hz, __shfl_xor(hz, off));
+  }
+  if (lane == 0) boxes[2 * t] = make_float4(lx, ly, lz, 0.0f);
+  if (lane == 1) boxes[2 * t + 1] = make_float4(hx, hy, hz, 0.0f);
+}
+
+// One wave per tile I; lane = atom i. Tiles J in increasing order, culled by
+// box distance; the atoms of a kept J are staged in LDS and tested in order.
+__global__ __launch_bounds__(64) void k_dense_list(const float* pos, int n, float r2, int K, const float4* boxes,
+                                                   int32_t* idx, int32_t* counts) {
+  __shared__ float4 pj[kTile];
+  const int I = blockIdx.x;
+  const int n_tiles = (n + kTile - 1) / kTile;
+  const int lane = threadIdx.x;
+  const int i = I * kTile + lane;
+  const bool vi = i < n;
+  const int ic = min(i, n - 1);
+  const float xi = pos[3 * ic], yi = pos[3 * ic + 1], zi = pos[3 * ic + 2];
+  const float4 li = boxes[2 * I], hi = boxes[2 * I + 1];
+  int cnt = 0;
+  for (int J = 0; J < n_tiles; ++J) {
+    const float4 lj = boxes[2 * J], hj = boxes[2 * J + 1];
+    const float bx = fmaxf(0.0f, fmaxf(lj.x - hi.x, li.x - hj.x));
+    const float by = fmaxf(0.0f, fmaxf(lj.y - hi.y, li.y - hj.y));
+    const float bz = fmaxf(0.0f, fmaxf(lj.z - hi.z, li.z - hj.z));
+    if (bx * bx + by * by + bz * bz >= r2 * 1.00001f) continue;  // wave-uniform
+    const int nJ = min(kTile, n - J * kTile);
+    const int j0 = J * kTile + min(lane, nJ - 1);
+    pj[lane] = make_float4(pos[3 * j0], pos[3 * j0 + 1], pos[3 * j0 + 2], 0.0f);
+    __syncthreads();
+    for (int b = 0; b < nJ; ++b) {
+      const float4 q = pj[b];
+      const float dx = q.x - xi, dy = q.y - yi, dz = q.z - zi;
+      const int j = J * kTile + b;
+      if (vi && j != i && dx * dx + dy * dy + dz * dz < r2) {
+        if (cnt < K) idx[(size_t)i * K + cnt] = j;
+        ++cnt;
+      }
+    }
+    __syncthreads();
+  }
+  if (!vi) return;
+  counts[i] = cnt;
+  for (int c = cnt; c < K; ++c) idx[(size_t)i * K + c] = n;
+}
+
 }  // namespace
+
+void launch_dense_list(const float* pos, int n, float radius, int capacity, int32_t* idx, int32_t* counts,
+                       float4* boxes, hipStream_t s) {
+  if (n <= 0) return;
+  const int n_tiles = (n + kTile - 1) / kTile;
+  const int wpb = kNlThreads / 64;
+  hipLaunchKernelGGL(k_dense_boxes, dim3((n_tiles + wpb - 1) / wpb), dim3(kNlThreads), 0, s, pos, n, boxes);
+  hipLaunchKernelGGL(k_dense_list, dim3(n_tiles), dim3(64), 0, s, pos, n, radius * radius, capacity, boxes, idx,
+                     counts);
+}
 
 void launch_nl_build(const DeviceView& v, const int32_t* slot_ranges, int n_slots, bool force, hipStream_t s) {
   if (n_slots <= 0) return;

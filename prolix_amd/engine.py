@@ -153,12 +153,14 @@ class GBEngine:
         the list at every evaluation, skin > 0 when an atom moved > skin / 2."""
         N.check(N.lib().prx_engine_set_nl_skin(self._h, float(skin)), "prx_engine_set_nl_skin")
 
-    STEP_MODES = {"auto": 0, "graph": 1, "persistent": 2}
+    STEP_MODES = {"auto": 0, "graph": 1, "persistent": 2, "fused": 3}
 
     def set_step_mode(self, mode: str):
-        """How steps run: "graph" (captured per-pass kernels), "persistent" (one
-        cooperative kernel per 1,000 steps, the same results bit for bit) or
-        "auto" (persistent for small engines)."""
+        """How steps run, with the same results bit for bit: "graph" (captured
+        per-pass kernels), "persistent" (one cooperative kernel per 1,000
+        steps, grid-wide barriers), "fused" (one workgroup per system,
+        workgroup barriers only; not for neighbour-list systems) or "auto"
+        (the default: graphs, the fastest of the three on MI355X)."""
         N.check(N.lib().prx_engine_set_step_mode(self._h, self.STEP_MODES[mode]), "prx_engine_set_step_mode")
 
     def set_fork_min_atoms(self, min_atoms: int):
@@ -245,7 +247,9 @@ class GBPool:
         self.sizes = {}
 
     def admit(self, sysd: dict, params: Langevin, positions=None, **opts) -> int:
-        """Upload one system, draw its momenta, compute its forces; returns its slot."""
+        """Upload one system; returns its slot. Its momenta and forces are
+        computed at the next step call or read, together with every other
+        system admitted since the last one."""
         d = N.Desc(sysd, **opts)
         x = np.ascontiguousarray(np.asarray(sysd["positions"] if positions is None else positions, np.float32))
         slot = ctypes.c_int32()
@@ -258,6 +262,16 @@ class GBPool:
         self._descs[slot.value] = d
         self.sizes[slot.value] = d.n_atoms
         return slot.value
+
+    def set_step_mode(self, mode: str):
+        """As GBEngine.set_step_mode, for the pool's live systems."""
+        N.check(N.lib().prx_engine_set_step_mode(self._h, GBEngine.STEP_MODES[mode]), "prx_engine_set_step_mode")
+
+    def set_step_limit(self, slot: int, limit: int):
+        """Stop the system after `limit` steps in all (0: never); later step
+        calls leave its state as it was after that step."""
+        N.check(N.lib().prx_pool_set_step_limit(self._h, int(slot), int(limit), _stream(self.torch)),
+                "prx_pool_set_step_limit")
 
     def release(self, slot: int):
         N.check(N.lib().prx_pool_release(self._h, int(slot), _stream(self.torch)), "prx_pool_release")

@@ -129,9 +129,17 @@ def fire_init(engine, positions, dt_start, alpha_start=0.1):
     return FireState(x, t.zeros_like(x), f.clone(), dt, alpha, n_pos)
 
 
-def fire_step(engine, state, masses, dt_max, force_cap=None, disp_cap=0.1, **kw):
-    """One FIRE step of every system (masses: packed (N,) tensor in amu)."""
+def fire_step(engine, state, masses, dt_max, force_cap=None, disp_cap=0.1, force_fn=None, cap_always=False,
+              disp_eps=1e-8, **kw):
+    """One FIRE step of every system (masses: packed (N,) tensor in amu).
+
+    ``force_fn(x)`` replaces the engine's forces (the batched minimiser adds
+    its restraints there); ``cap_always`` applies the displacement cap without
+    a force cap (batched_simulate.py:843-856, |dr| regularised by
+    ``disp_eps``)."""
     t = engine.torch
+    if force_fn is None:
+        force_fn = lambda x: engine.energy_forces(x, energies=False)[1].clone()
     o = dict(FIRE_DEFAULTS, **kw)
     sid = t.as_tensor(np.repeat(np.arange(engine.n_systems), engine.sizes), device=state.positions.device)
     ns = engine.n_systems
@@ -144,8 +152,7 @@ def fire_step(engine, state, masses, dt_max, force_cap=None, disp_cap=0.1, **kw)
     dt = state.dt[sid, None].to(x0.dtype)
     p = p + 0.5 * dt * f
     x = x0 + dt * p / m
-    _, f_new, _ = engine.energy_forces(x, energies=False)
-    f_new = f_new.clone()
+    f_new = force_fn(x)
     p = p + 0.5 * dt * f_new
 
     lengths = _lengths(engine)
@@ -167,10 +174,13 @@ def fire_step(engine, state, masses, dt_max, force_cap=None, disp_cap=0.1, **kw)
     alpha = t.where(grow, alpha * o["f_alpha"], alpha)
     alpha = t.where(fdotp < 0, t.full_like(alpha, o["alpha_start"]), alpha)
     p = p * (fdotp >= 0)[sid, None].to(p.dtype)
-    if force_cap is not None and disp_cap is not None:
+    if (force_cap is not None or cap_always) and disp_cap is not None:
         # as the reference, the state keeps the force of the uncapped positions
         dr = x - x0
-        s = t.clamp(disp_cap / (t.linalg.vector_norm(dr, dim=1, keepdim=True) + 1e-8), max=1.0)
+        if cap_always:
+            s = t.clamp(disp_cap / t.sqrt((dr * dr).sum(1, keepdim=True) + disp_eps), max=1.0)
+        else:
+            s = t.clamp(disp_cap / (t.linalg.vector_norm(dr, dim=1, keepdim=True) + disp_eps), max=1.0)
         x = x0 + dr * s
         p = p * s
     return FireState(x, p, f_new, dt_s, alpha, n_pos)
@@ -210,3 +220,197 @@ def staged_fire(engine, positions, masses, stages=FIRE_STAGES):
         dt_start = min(dt_start * 2.0, 0.001)
         dt_max = min(dt_start * 10.0, 0.01)
     return x
+
+
+# ----------------------------------------------------------------------------
+# The batched minimiser's later stages (batched_simulate.py:681-942): four
+# restrained FIRE sub-stages that keep their best snapshot, then an
+# unrestrained L-BFGS polish with its rollback, then the convergence gate.
+# ----------------------------------------------------------------------------
+
+FIRE_RESTRAINTS = (500.0, 125.0, 25.0, 0.0)  # kcal/mol/A^2; backbone, then none (batched_simulate.py:802-803)
+
+
+def _sid(engine, device):
+    return engine.torch.as_tensor(np.repeat(np.arange(engine.n_systems), engine.sizes), device=device)
+
+
+def _rms_force(engine, f):
+    """Per-system sqrt(mean over atoms of |F|^2) (batched_simulate.py:826-828)."""
+    t = engine.torch
+    lengths = _lengths(engine)
+    ss = t.segment_reduce((f.to(t.float64) ** 2).sum(1), "sum", lengths=lengths)
+    return t.sqrt(ss / lengths.to(t.float64))
+
+
+def restrained_fire(engine, positions, masses, steps, k_restraint=0.0, selection=None, reference=None,
+                    dt_start=1e-4, dt_max=1e-3, disp_cap=0.1):
+    """One FIRE sub-stage of batched_minimize (batched_simulate.py:801-874).
+
+    Forces are the engine's plus a harmonic restraint -k (r - r_ref) on the
+    selected atoms (r_ref: ``reference``, default the start), non-finite
+    components set to 0 but magnitudes never capped; each step's displacement
+    is capped at ``disp_cap`` per atom with the momentum scaled alike. Each
+    system keeps the positions of its lowest RMS force seen before a step
+    and returns them. Returns (best positions, best RMS force per system)."""
+    t = engine.torch
+    x0 = engine._pos_tensor(positions).clone()
+    ref = x0 if reference is None else engine._pos_tensor(reference)
+    sel = None
+    if k_restraint > 0.0:
+        sel = t.ones(engine.n_atoms, dtype=x0.dtype, device=x0.device) if selection is None else \
+            t.as_tensor(np.asarray(selection, np.float32).reshape(-1), device=x0.device)
+
+    def force_fn(x):
+        f = engine.energy_forces(x, energies=False)[1].clone()
+        if sel is not None:
+            f = f - float(k_restraint) * (x - ref) * sel[:, None]
+        return t.where(t.isfinite(f), f, t.zeros_like(f))
+
+    mt = t.as_tensor(np.asarray(masses, np.float64).reshape(-1), dtype=t.float32, device=engine.device)
+    ns = engine.n_systems
+    st = FireState(x0, t.zeros_like(x0), force_fn(x0), t.full((ns,), float(dt_start), dtype=t.float64, device=x0.device),
+                   t.full((ns,), 0.1, dtype=t.float64, device=x0.device), t.zeros((ns,), dtype=t.int64, device=x0.device))
+    sid = _sid(engine, x0.device)
+    best_x, best_rms = x0.clone(), t.full((ns,), 1e10, dtype=t.float64, device=x0.device)
+    for _ in range(int(steps)):
+        rms = _rms_force(engine, st.forces)
+        better = rms < best_rms
+        best_x = t.where(better[sid, None], st.positions, best_x)
+        best_rms = t.where(better, rms, best_rms)
+        st = fire_step(engine, st, mt, dt_max, None, disp_cap, force_fn=force_fn, cap_always=True, disp_eps=1e-30)
+    return best_x, best_rms
+
+
+def _energy(engine, x):
+    t = engine.torch
+    e, _, _ = engine.energy_forces(x, energies=True)
+    e = e[:, 11]
+    return t.where(t.isfinite(e), e, t.full_like(e, 1e10))
+
+
+def lbfgs(engine, positions, steps=2000, history=10, max_stepsize=0.1, tol=1e-6, c1=1e-4, shrink=0.5,
+          max_trials=12):
+    """Unrestrained L-BFGS polish of every system (the last stage of
+    batched_simulate.py:886-935, which runs jaxopt.LBFGS with history 10,
+    max_stepsize 0.1 and a backtracking line search on E and its gradient).
+
+    Per system: the two-loop recursion (Nocedal & Wright, Numerical
+    Optimization, Alg. 7.4) with H0 = (s.y / y.y) I over the last ``history``
+    pairs with s.y > 0, then a backtracking search from t = max_stepsize,
+    t <- shrink t, accepting the first t with E(x + t d) <= E(x) + c1 t g.d
+    (Armijo); a system whose search fails, or whose gradient norm is below
+    ``tol``, stays put for that iteration. Non-finite energies count as 1e10
+    and non-finite gradient components as 0, as the reference's objective.
+    jaxopt's own line-search details are not restated: the iterates follow
+    this textbook method, not jaxopt's bit for bit."""
+    t = engine.torch
+    x = engine._pos_tensor(positions).clone().to(t.float64)
+    sid = _sid(engine, x.device)
+    lengths = _lengths(engine)
+    ns = engine.n_systems
+
+    def dot(a, b):
+        return t.segment_reduce((a * b).sum(1), "sum", lengths=lengths)
+
+    def grad(xx):
+        g = -engine.energy_forces(xx.to(t.float32), energies=False)[1].to(t.float64)
+        return t.where(t.isfinite(g), g, t.zeros_like(g))
+
+    e = _energy(engine, x.to(t.float32)).to(t.float64)
+    g = grad(x)
+    S, Y, R = [], [], []  # history pairs and 1/(s.y), per system (zero where unused)
+    for _ in range(int(steps)):
+        gn = t.sqrt(dot(g, g))
+        active = gn > tol
+        if not bool(active.any()):
+            break
+        q = g.clone()
+        alphas = []
+        for s_, y_, r_ in reversed(list(zip(S, Y, R))):
+            a = r_ * dot(s_, q)
+            alphas.append(a)
+            q = q - a[sid, None] * y_
+        if S:
+            sy, yy = dot(S[-1], Y[-1]), dot(Y[-1], Y[-1])
+            gamma = t.where((sy > 0) & (yy > 0), sy / t.clamp(yy, min=1e-300), t.ones_like(sy))
+        else:
+            gamma = t.ones(ns, dtype=t.float64, device=x.device)
+        r = gamma[sid, None] * q
+        for (s_, y_, r_), a in zip(zip(S, Y, R), reversed(alphas)):
+            b = r_ * dot(y_, r)
+            r = r + (a - b)[sid, None] * s_
+        d = -r
+        gd = dot(g, d)
+        d = t.where((gd < 0)[sid, None], d, -g)  # not a descent direction: steepest descent
+        gd = dot(g, d)
+        step = t.zeros(ns, dtype=t.float64, device=x.device)
+        e_new = e.clone()
+        pending = active.clone()
+        tt = float(max_stepsize)
+        for _k in range(int(max_trials)):
+            xt = x + (tt * pending.to(t.float64))[sid, None] * d
+            et = _energy(engine, xt.to(t.float32)).to(t.float64)
+            ok = pending & (et <= e + c1 * tt * gd)
+            step = t.where(ok, t.full_like(step, tt), step)
+            e_new = t.where(ok, et, e_new)
+            pending = pending & ~ok
+            if not bool(pending.any()):
+                break
+            tt *= shrink
+        moved = step > 0
+        x_new = x + step[sid, None] * d
+        g_new = grad(x_new)
+        s_v, y_v = x_new - x, g_new - g
+        sy = dot(s_v, y_v)
+        keep = moved & (sy > 1e-300)
+        rho = t.where(keep, 1.0 / t.where(keep, sy, t.ones_like(sy)), t.zeros_like(sy))
+        S.append(s_v * keep[sid, None].to(s_v.dtype))
+        Y.append(y_v * keep[sid, None].to(y_v.dtype))
+        R.append(rho)
+        if len(S) > history:
+            S.pop(0), Y.pop(0), R.pop(0)
+        x, g, e = x_new, t.where(moved[sid, None], g_new, g), e_new
+    return x.to(t.float32)
+
+
+def batched_minimize(engine, positions, masses, heavy=None, backbone=None, max_steps=5000, lbfgs_steps=2000,
+                     fire_stage_steps=(500, 500, 500, 500), min_convergence=1e-4):
+    """The six-stage batched minimiser (batched_simulate.py:681-942):
+
+    1. capped steepest descent, 500 kcal/mol/A^2 on the heavy atoms,
+       0.01 A cap, ``max_steps`` steps, rolled back where non-finite;
+    2-5. restrained FIRE on the backbone at 500, 125, 25 kcal/mol/A^2, then
+       unrestrained, each stage from the previous stage's best snapshot;
+       non-finite results fall back to the SD positions;
+    6. L-BFGS, rolled back to the FIRE result where an atom leaves |x| <= 500 A.
+
+    ``heavy`` / ``backbone``: packed 0/1 atom flags (default: every atom).
+    Returns (positions, converged (n_systems,) bool, rms force per system)."""
+    t = engine.torch
+    if len(fire_stage_steps) != 4:
+        raise ValueError(f"fire_stage_steps must have 4 values, got {len(fire_stage_steps)}")
+    x_ref = engine._pos_tensor(positions).clone()
+    sid = _sid(engine, x_ref.device)
+    lengths = _lengths(engine)
+
+    def per_system_ok(x):
+        return t.segment_reduce(t.isfinite(x).all(1).to(t.float64), "min", lengths=lengths) > 0.5
+
+    x_sd = capped_descent(engine, x_ref, steps=max_steps, step_size=0.002, max_dx=0.01, k_restraint=500.0,
+                          restrained=heavy)
+    x = x_sd
+    for k_r, n in zip(FIRE_RESTRAINTS, fire_stage_steps):
+        sel = backbone if k_r > 0 else None
+        x, _ = restrained_fire(engine, x, masses, n, k_restraint=k_r, selection=sel, reference=x_ref)
+    x = t.where(per_system_ok(x)[sid, None], x, x_sd)
+    x_fire = x
+    if lbfgs_steps > 0:
+        xl = lbfgs(engine, x_fire, steps=lbfgs_steps)
+        big = t.segment_reduce(t.abs(xl).amax(1).to(t.float64), "max", lengths=lengths)
+        diverged = ~(big <= 500.0)
+        x = t.where(diverged[sid, None], x_fire, xl)
+    _, f, _ = engine.energy_forces(x, energies=False)
+    rms = t.sqrt(t.segment_reduce((f.to(t.float64) ** 2).sum(1) + 1e-30, "sum", lengths=lengths)
+                 / lengths.to(t.float64))
+    return x, rms < min_convergence, rms

@@ -5,6 +5,9 @@ Mirrors prolix's ``physics/neighbor_list.py``:
 - ``ExclusionSpec`` / ``ExclusionSpec.from_protein``  (:25-142)
 - ``max_exclusion_slots_needed``                     (:208-233)
 - ``map_exclusions_to_dense_padded``                 (:236-303)
+- ``make_neighbor_list_fn``                          (:155-172)
+- ``get_neighbor_exclusion_scales``                  (:382-448)
+- ``compute_nl_capacity``                            (:451-530)
 
 One deliberate difference: the reference silently truncates an atom's row at
 ``max_exclusions`` (:297); here that raises ``ValueError`` instead, because a
@@ -128,3 +131,145 @@ def map_exclusions_to_dense_padded(spec: ExclusionSpec, max_exclusions: int = 32
             sv[i, k] = v
             se[i, k] = e
     return idx, sv, se
+
+
+# ---------------------------------------------------------------------------
+# Neighbour-list object and its helpers (neighbor_list.py:155-172, 382-530)
+# ---------------------------------------------------------------------------
+
+def get_neighbor_exclusion_scales(excl_indices, excl_scales_vdw, excl_scales_elec, neighbor_idx):
+    """(scale_vdw, scale_elec), each (N, K): the exclusion scales of every listed
+    neighbour, 1 where the pair is not in the atom's exclusion row
+    (neighbor_list.py:382-448: columns m = 0..M-1 in order, a later matching
+    column overwriting an earlier one). Works on NumPy arrays or torch tensors
+    (on their device)."""
+    is_torch = hasattr(neighbor_idx, "device") and hasattr(neighbor_idx, "dtype") and not isinstance(
+        neighbor_idx, np.ndarray)
+    if is_torch:
+        import torch
+
+        dev = neighbor_idx.device
+        ex = torch.as_tensor(excl_indices, device=dev)
+        sv = torch.as_tensor(excl_scales_vdw, dtype=torch.float32, device=dev)
+        se = torch.as_tensor(excl_scales_elec, dtype=torch.float32, device=dev)
+        out_v = torch.ones(neighbor_idx.shape, dtype=torch.float32, device=dev)
+        out_e = torch.ones_like(out_v)
+        for m in range(ex.shape[1]):
+            col = ex[:, m:m + 1]
+            hit = (neighbor_idx == col) & (col != -1)
+            out_v = torch.where(hit, sv[:, m:m + 1], out_v)
+            out_e = torch.where(hit, se[:, m:m + 1], out_e)
+        return out_v, out_e
+    nb = np.asarray(neighbor_idx)
+    ex = np.asarray(excl_indices)
+    sv = np.asarray(excl_scales_vdw, np.float32)
+    se = np.asarray(excl_scales_elec, np.float32)
+    out_v = np.ones(nb.shape, np.float32)
+    out_e = np.ones(nb.shape, np.float32)
+    for m in range(ex.shape[1]):
+        col = ex[:, m:m + 1]
+        hit = (nb == col) & (col != -1)
+        out_v = np.where(hit, sv[:, m:m + 1], out_v)
+        out_e = np.where(hit, se[:, m:m + 1], out_e)
+    return out_v, out_e
+
+
+def compute_nl_capacity(atom_bucket_size: int, box_size, cutoff: float, dr_threshold: float = 0.5,
+                        safety_factor: float = 1.5, round_to: int = 256) -> int:
+    """Static list capacity of a shape class (neighbor_list.py:451-530):
+    ceil_to(round_to, safety_factor * (N / V) * 4/3 pi (cutoff + dr_threshold)^3)
+    for a periodic box of volume V; ValueError for a non-positive volume."""
+    import math
+
+    vol = float(np.prod(np.asarray(box_size, np.float64)))
+    if vol <= 0:
+        raise ValueError(f"compute_nl_capacity: box_size must describe a positive-volume periodic box, "
+                         f"got {box_size}")
+    k_raw = safety_factor * (atom_bucket_size / vol) * (4.0 / 3.0) * math.pi * (cutoff + dr_threshold) ** 3
+    return int(math.ceil(k_raw / round_to) * round_to)
+
+
+@dataclasses.dataclass
+class NeighborList:
+    """A jax_md-style Dense neighbour list: ``idx`` (N, K) int32 device tensor
+    with N after each row's last neighbour, built at ``reference_position``."""
+
+    idx: object
+    reference_position: object
+    did_buffer_overflow: bool
+    max_occupancy: int
+    cutoff: float
+    dr_threshold: float
+    update_fn: object = None
+
+    def update(self, positions, **kwargs) -> "NeighborList":
+        return self.update_fn(positions, self, **kwargs)
+
+
+class NeighborListFns:
+    """``allocate`` / ``update`` of make_neighbor_list_fn."""
+
+    def __init__(self, cutoff: float, dr_threshold: float, capacity_multiplier: float):
+        self.cutoff = float(cutoff)
+        self.dr_threshold = float(dr_threshold)
+        self.capacity_multiplier = float(capacity_multiplier)
+
+    def _build(self, positions, capacity: int | None):
+        import ctypes
+
+        import torch
+
+        from prolix_amd import _native as N
+
+        x = torch.as_tensor(positions, dtype=torch.float32, device="cuda").reshape(-1, 3).contiguous()
+        n = x.shape[0]
+        counts = torch.empty((n,), dtype=torch.int32, device="cuda")
+        work = torch.empty((8 * ((n + 63) // 64),), dtype=torch.float32, device="cuda")
+        stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        radius = self.cutoff + self.dr_threshold
+
+        def run(k):
+            idx = torch.empty((n, k), dtype=torch.int32, device="cuda")
+            N.check(N.lib().prx_neighbor_list_dense(ctypes.c_void_p(x.data_ptr()), n, radius, k,
+                                                    ctypes.c_void_p(idx.data_ptr()),
+                                                    ctypes.c_void_p(counts.data_ptr()),
+                                                    ctypes.c_void_p(work.data_ptr()), stream),
+                    "prx_neighbor_list_dense")
+            return idx
+
+        if capacity is None:  # size from the occupancy, as jax_md's allocate does
+            run(1)
+            occ = int(counts.max().item()) if n else 0
+            capacity = max(1, int(occ * self.capacity_multiplier))
+        idx = run(capacity)
+        occ = int(counts.max().item()) if n else 0
+        return NeighborList(idx, x.clone(), occ > capacity, occ, self.cutoff, self.dr_threshold, self.update)
+
+    def allocate(self, positions, extra_capacity: int = 0, **kwargs) -> NeighborList:
+        nb = self._build(positions, None)
+        if extra_capacity:
+            nb = self._build(positions, nb.idx.shape[1] + int(extra_capacity))
+        return nb
+
+    def update(self, positions, neighbors: NeighborList, **kwargs) -> NeighborList:
+        """Rebuild at the same capacity when an atom moved more than
+        dr_threshold / 2 since the list was built; otherwise keep it."""
+        import torch
+
+        x = torch.as_tensor(positions, dtype=torch.float32, device="cuda").reshape(-1, 3)
+        moved = torch.linalg.vector_norm(x - neighbors.reference_position, dim=1).max().item()
+        if not moved > 0.5 * self.dr_threshold:
+            return neighbors
+        return self._build(x, neighbors.idx.shape[1])
+
+
+def make_neighbor_list_fn(displacement_fn, box_size, cutoff: float, capacity_multiplier: float = 1.25,
+                          disable_cell_list: bool = False, dr_threshold: float = 0.5) -> NeighborListFns:
+    """Neighbour-list functions (neighbor_list.py:155-172): a Dense list of the
+    pairs within cutoff + dr_threshold (0.5 A, the reference's skin), without
+    self pairs, rebuilt on ``update`` when an atom moved more than half the
+    skin. Free space only: ``box_size`` is not used for wrapping (the engine
+    has no periodic boxes). The list is built on the GPU (tile boxes, then
+    per-atom scans of the tiles in range); ``disable_cell_list`` has no
+    effect."""
+    return NeighborListFns(cutoff, dr_threshold, capacity_multiplier)

@@ -77,11 +77,26 @@ def _bonded(sysd) -> int:
 
 
 def serve(requests, hbm_budget_bytes: float = 200e9, max_systems: int = 1024, chunk_steps: int = 100,
-          max_atoms: int | None = None, sample_energies=(), safety_caps: bool = True, clock=None, log=None):
-    """Run the request trace to completion; returns (results, stats)."""
+          max_atoms: int | None = None, sample_energies=(), safety_caps: bool = True, clock=None, log=None,
+          chunk_s: float | None = None):
+    """Run the request trace to completion; returns (results, stats).
+
+    Each admitted system carries its step count as a device step limit
+    (GBPool.set_step_limit): a chunk may run past the end of a request without
+    changing its result, so chunks need not stop at the next request's end.
+    After each chunk the finished (and failed) systems are read back and
+    evicted and the arrived requests admitted; the pool computes the new
+    systems' momenta and forces together at the next chunk.
+
+    chunk_steps: steps per chunk, or with chunk_s the longest chunk; chunk_s:
+    the target wall time of a chunk, whose length then follows the step rate
+    of the chunk before it (in multiples of 50 steps, the engine's graph).
+    """
     import torch
 
     reqs = sorted(requests, key=lambda r: r.arrival)
+    if any(r.n_steps < 1 for r in reqs):
+        raise ValueError("every request needs n_steps >= 1")
     if max_atoms is None:
         max_atoms = max(1, int(min(sum(len(r.system["charges"]) for r in reqs), 8_000_000)))
     pool = GBPool(max_atoms=max_atoms, max_systems=max_systems)
@@ -94,10 +109,15 @@ def serve(requests, hbm_budget_bytes: float = 200e9, max_systems: int = 1024, ch
     used = 0
     pair_steps = 0.0
     sys_steps = 0
+    chunks = 0
+    chunk = chunk_steps
+    steady = None  # (wall, system-steps, pair-steps) when the last request was admitted
+    t_admit = t_launch = t_wait = t_evict = 0.0  # host wall time spent in each phase
     sample = set(sample_energies)
     while waiting or live:
         t = now()
         # admit arrived requests in arrival order while the budget allows
+        ta = clock()
         while waiting and waiting[0].arrival <= t and len(live) < max_systems:
             r = waiting[0]
             cost = hbm_bytes(len(r.system["charges"]), _bonded(r.system))
@@ -108,22 +128,33 @@ def serve(requests, hbm_budget_bytes: float = 200e9, max_systems: int = 1024, ch
                 if not live:
                     raise RuntimeError(f"request {r.rid} ({len(r.system['charges'])} atoms) does not fit the pool")
                 break
+            pool.set_step_limit(slot, r.n_steps)
             waiting.pop(0)
             used += cost
             live[slot] = [r, r.n_steps, now(), cost]
+        t_admit += clock() - ta
+        if steady is None and not waiting:
+            steady = (now(), sys_steps, pair_steps)
         if not live:
             wait = waiting[0].arrival - now()
             if wait > 0:
                 time.sleep(min(wait, 0.05))
             continue
-        chunk = min(chunk_steps, min(v[1] for v in live.values()))
+        ts = clock()
         pool.steps(chunk)
+        tl = clock()
         torch.cuda.synchronize()
+        tw = clock()
+        t_launch += tl - ts
+        t_wait += tw - tl
+        chunks += 1
+        te = clock()
         for v in live.values():
-            v[1] -= chunk
+            k = min(chunk, v[1])  # steps the system took (it stops at its limit)
+            v[1] -= k
             n = len(v[0].system["charges"])
-            pair_steps += chunk * n * (n - 1) / 2.0
-            sys_steps += chunk
+            pair_steps += k * n * (n - 1) / 2.0
+            sys_steps += k
         # explosion check on the device counters: a system whose positions or
         # momenta turned non-finite is evicted with an error result; the others
         # never read its atoms, so they continue unchanged
@@ -143,18 +174,34 @@ def serve(requests, hbm_budget_bytes: float = 200e9, max_systems: int = 1024, ch
                 log(f"request {r.rid}: {len(r.system['charges'])} atoms, {steps} steps, "
                     f"latency {results[-1].latency:.2f}s, live {len(live)}, waiting {len(waiting)}"
                     + (f", evicted: {err}" if err else ""))
+        t_evict += clock() - te
+        if chunk_s is not None:
+            rate = chunk / max(tw - ts, 1e-6)  # steps per second of the last chunk
+            chunk = int(min(chunk_steps, max(50, 50 * round(rate * chunk_s / 50.0))))
     wall = now()
     pool.close()
     lat = np.array([r.latency for r in results])
+    st_wall, st_sys, st_pair = steady if steady is not None else (wall, sys_steps, pair_steps)
     stats = {
         "requests": len(results),
         "failed": sum(r.error is not None for r in results),
         "wall_s": wall,
+        "chunks": chunks,
         "latency_p50_s": float(np.percentile(lat, 50)),
         "latency_p99_s": float(np.percentile(lat, 99)),
         "aggregate_ns_per_day": sys_steps * 1e-6 * 86400.0 / wall * (reqs[0].dt_fs if reqs else 1.0),
         "pair_interactions_per_s": pair_steps / wall,
         "offered_pair_steps": float(sum(len(r.system["charges"]) ** 2 / 2.0 * r.n_steps for r in reqs)),
+        # up to the admission of the last request: while requests were still
+        # waiting, so the pool held as many systems as the limits allow
+        "until_last_admission": {"wall_s": st_wall,
+                                 "aggregate_ns_per_day": st_sys * 1e-6 * 86400.0 / max(st_wall, 1e-9)
+                                 * (reqs[0].dt_fs if reqs else 1.0),
+                                 "pair_interactions_per_s": st_pair / max(st_wall, 1e-9)},
+        # admit: uploads; launch: momenta, forces, work lists and graphs of the
+        # admitted systems plus the enqueue of the chunk; wait: until the chunk
+        # ends; evict: counters, read-back and release
+        "host_time_s": {"admit": t_admit, "launch": t_launch, "wait": t_wait, "evict": t_evict},
     }
     return sorted(results, key=lambda r: r.rid), stats
 

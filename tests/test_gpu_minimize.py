@@ -83,3 +83,73 @@ def test_capped_descent_matches_oracle(k):
     # into the restraint force each step; without it the bound is 1e-4 A
     assert err < (5e-4 if k > 0 else 1e-4)
     assert O.total_energy(s, xg) < O.total_energy(s, s["positions"])
+
+
+def test_restrained_fire_stage_matches_oracle():
+    """One restrained FIRE sub-stage of the batched minimiser
+    (batched_simulate.py:813-874): restraint 500 kcal/mol/A^2 on every third
+    atom toward the start, 0.1 A step cap, best snapshot."""
+    from prolix_amd.engine import GBEngine
+    from prolix_amd.minimize import restrained_fire
+
+    s = protein(60, 4)
+    m = np.asarray(s["masses"], np.float64)
+    x0 = np.asarray(s["positions"], np.float64)
+    sel = (np.arange(60) % 3 == 0).astype(np.float64)
+    ff = lambda x: O.forces(s, x) - 500.0 * (x - x0) * sel[:, None]
+    print("[gpu] restrained FIRE stage, 30 steps", flush=True)
+    eng = GBEngine(s)
+    xg, rg = restrained_fire(eng, x0, m, 30, k_restraint=500.0, selection=sel)
+    xg, rg = xg.cpu().numpy().astype(np.float64), float(rg.cpu().numpy()[0])
+    eng.close()
+    xo, ro = O.fire_stage_best(ff, x0, m, 30)
+    err = float(np.max(np.abs(xg - xo)))
+    print(f"[fire stage] max |dx| {err:.2e} A, best rms device {rg:.5f} oracle {ro:.5f}", flush=True)
+    assert err < 1e-3
+    assert abs(rg - ro) <= 1e-4 * ro
+
+
+def test_lbfgs_matches_oracle():
+    """The L-BFGS polish: 10 iterations of the same method in float64."""
+    from prolix_amd.engine import GBEngine
+    from prolix_amd.minimize import lbfgs
+
+    s = protein(60, 4)
+    print("[gpu] L-BFGS 10 iterations", flush=True)
+    eng = GBEngine(s)
+    xg = lbfgs(eng, s["positions"], steps=10).cpu().numpy().astype(np.float64)
+    eng.close()
+    xo, tr = O.lbfgs_minimize(lambda x: O.total_energy(s, x), lambda x: -O.forces(s, x), s["positions"], 10)
+    e0, eg, eo = O.total_energy(s, s["positions"]), O.total_energy(s, xg), O.total_energy(s, xo)
+    err = float(np.max(np.abs(xg - xo)))
+    print(f"[lbfgs] max |dx| {err:.2e} A; E {e0:.3f} -> device {eg:.4f}, oracle {eo:.4f}", flush=True)
+    assert np.all(np.diff(tr) <= 0) and eo < e0
+    assert err < 2e-3
+    assert abs(eg - eo) <= 1e-5 * abs(e0)
+
+
+def test_batched_minimize_stages_per_system():
+    """The six stages on two systems together; each equals its run alone to
+    float32 rounding, and every system ends below its starting energy."""
+    from prolix_amd.engine import GBEngine
+    from prolix_amd.minimize import batched_minimize
+
+    systems = [protein(n, 60 + k, radius=synth.compact_radius(n)) for k, n in enumerate((80, 150))]
+    masses = np.concatenate([np.asarray(s["masses"], np.float64) for s in systems])
+    kw = dict(max_steps=40, lbfgs_steps=15, fire_stage_steps=(20, 20, 20, 20))
+    print("[gpu] batched minimiser, 2 systems", flush=True)
+    eng = GBEngine(systems)
+    x, conv, rms = batched_minimize(eng, [s["positions"] for s in systems], masses, **kw)
+    xb = eng.unpack(x)
+    e_start = eng.energy_forces([s["positions"] for s in systems])[0].cpu().numpy()[:, -1]
+    e_end = eng.energy_forces(x)[0].cpu().numpy()[:, -1]
+    eng.close()
+    assert np.all(e_end < e_start) and np.isfinite(xb[0]).all() and conv.shape == (2,)
+    off = 0
+    for k, s in enumerate(systems):
+        n = len(s["charges"])
+        one = GBEngine(s)
+        xa, _, ra = batched_minimize(one, s["positions"], masses[off:off + n], **kw)
+        one.close()
+        off += n
+        assert np.max(np.abs(xa.cpu().numpy() - xb[k])) < 1e-4

@@ -106,3 +106,44 @@ def test_list_systems_packed_equal_alone():
         one.close()
         sl = slice(off[k], off[k + 1])
         assert np.array_equal(xa, x[sl]) and np.array_equal(pa, p[sl]) and np.array_equal(fa, f[sl])
+
+
+def test_dense_neighbor_list_object():
+    """make_neighbor_list_fn (neighbor_list.py:155-172): every pair within
+    cutoff + 0.5 A, both directions, no self pairs, rows in increasing index,
+    the atom count as the sentinel; update keeps the list below half the skin
+    and rebuilds above it; a list too small reports the overflow."""
+    import torch
+    from scipy.spatial import cKDTree
+
+    from prolix_amd.physics.neighbor_list import make_neighbor_list_fn
+
+    s = protein(2500, 13, radius=synth.compact_radius(2500))
+    x = np.asarray(s["positions"], np.float32)
+    n = len(x)
+    rc = 9.0
+    print("[gpu] dense neighbour list, 2500 atoms, 9 + 0.5 A", flush=True)
+    fns = make_neighbor_list_fn(None, None, rc)
+    nb = fns.allocate(x)
+    idx = nb.idx.cpu().numpy()
+    assert not nb.did_buffer_overflow and idx.shape[1] >= nb.max_occupancy
+    # the exact float32 test of the kernel, away from the boundary
+    x64 = x.astype(np.float64)
+    r = rc + 0.5
+    tree = cKDTree(x64)
+    sure = tree.query_ball_point(x64, r - 1e-4)
+    maybe = tree.query_ball_point(x64, r + 1e-4)
+    for i in range(n):
+        row = idx[i][idx[i] < n]
+        assert np.all(np.diff(row) > 0) and i not in row and np.all(idx[i][len(row):] == n)
+        got = set(row.tolist())
+        assert set(sure[i]) - {i} <= got <= set(maybe[i])
+    same = fns.update(x + 0.1, nb)  # every atom moved 0.17 A < dr_threshold / 2 = 0.25 A: kept
+    assert same is nb
+    moved = x.copy()
+    moved[7] += 0.3
+    nb2 = fns.update(moved, nb)
+    assert nb2 is not nb and nb2.idx.shape == nb.idx.shape
+    small = fns._build(x, 8)  # capacity 8: overflow reported
+    assert small.did_buffer_overflow and small.max_occupancy == nb.max_occupancy
+    torch.cuda.synchronize()

@@ -116,6 +116,31 @@ def test_pool_result_is_the_same_alone_and_in_the_mix():
     assert np.array_equal(xa, xm) and np.array_equal(pa, pm) and np.array_equal(fa, fm)
 
 
+def test_step_limit_stops_one_system_exactly():
+    """A system with a step limit ends in the state it reaches alone after that
+    many steps, however far the pool steps on; the others are unaffected."""
+    from prolix_amd.engine import GBPool, Langevin
+
+    systems = [protein(n, 160 + k, radius=synth.compact_radius(n)) for k, n in enumerate((300, 520))]
+    params = [Langevin(1.0, 300.0, 1.0, seed=700 + k, safety_caps=True) for k in range(2)]
+    print("[gpu] pool step limit", flush=True)
+    alone = _pool_run(systems, [("admit", 0), ("steps", 70), ("read", 0)], params)
+    pool = GBPool(max_atoms=2000, max_systems=4)
+    slots = [pool.admit(s, p) for s, p in zip(systems, params)]
+    pool.set_step_limit(slots[0], 70)
+    pool.steps(50)
+    pool.steps(100)  # runs past the limit of system 0
+    x0, p0, f0, n0 = pool.read(slots[0])
+    x1, p1, f1, n1 = pool.read(slots[1])
+    pool.close()
+    other = _pool_run(systems, [("admit", 1), ("steps", 150), ("read", 1)], params)
+    xa, pa, fa, na = alone[0]
+    assert n0 == na == 70 and n1 == 150
+    assert np.array_equal(x0, xa) and np.array_equal(p0, pa) and np.array_equal(f0, fa)
+    xb, pb, fb, nb = other[1]
+    assert np.array_equal(x1, xb) and np.array_equal(p1, pb) and np.array_equal(f1, fb)
+
+
 def test_serving_loop_small_trace():
     from prolix_amd.serving import poisson_trace, serve
 

@@ -218,3 +218,38 @@ def test_cpu_restatement_float32_cutoff_classification():
     b, fb = C.CpuSystem(sysd, cutoff=cutoff, gb_cutoff=gbc).energy_forces(sysd["positions"])
     # (the OpenMP restatement's sums are not in a fixed order: equal to rounding)
     assert np.allclose(a, b, rtol=1e-13, atol=1e-10) and np.allclose(fa, fb, rtol=1e-12, atol=1e-11)
+
+
+def test_neighbor_exclusion_scales_follow_the_rows():
+    """get_neighbor_exclusion_scales (neighbor_list.py:382-448) against a loop
+    over each listed pair; a later matching column overwrites an earlier one."""
+    from prolix_amd.physics.neighbor_list import get_neighbor_exclusion_scales
+
+    rng = np.random.default_rng(3)
+    n, M, K = 40, 6, 9
+    ex = np.full((n, M), -1, np.int32)
+    sv = rng.uniform(0, 1, (n, M)).astype(np.float32)
+    se = rng.uniform(0, 1, (n, M)).astype(np.float32)
+    for i in range(n):
+        m = rng.integers(0, M + 1)
+        ex[i, :m] = rng.choice(n, size=m, replace=True)
+    nb = rng.integers(0, n + 1, size=(n, K)).astype(np.int32)  # n = the sentinel
+    gv, ge = get_neighbor_exclusion_scales(ex, sv, se, nb)
+    for i in range(n):
+        for k in range(K):
+            v = e = 1.0
+            for m in range(M):
+                if ex[i, m] != -1 and ex[i, m] == nb[i, k]:
+                    v, e = sv[i, m], se[i, m]
+            assert gv[i, k] == np.float32(v) and ge[i, k] == np.float32(e)
+
+
+def test_nl_capacity_matches_the_documented_case():
+    """compute_nl_capacity's own worked example (neighbor_list.py:476-483): the
+    5,000-atom, 32 A box, 9 A cutoff class gives 822 before rounding, 1024 after."""
+    from prolix_amd.physics.neighbor_list import compute_nl_capacity
+
+    assert compute_nl_capacity(5000, [32.0, 32.0, 32.0], 9.0, round_to=1) == 822  # ceil(821.998)
+    assert compute_nl_capacity(5000, [32.0, 32.0, 32.0], 9.0) == 1024
+    with pytest.raises(ValueError):
+        compute_nl_capacity(100, [0.0, 10.0, 10.0], 9.0)

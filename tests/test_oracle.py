@@ -499,3 +499,42 @@ def test_sd_backtrack_in_a_bowl():
     # at the minimum nothing is lower than E(r): r stays
     x1, frac, _, _ = O.sd_backtrack_step(efn, ffn, np.zeros((2, 3)))
     assert frac is None and np.array_equal(x1, np.zeros((2, 3)))
+
+
+def test_fire_stage_best_in_a_bowl():
+    """The restrained FIRE sub-stage (batched_simulate.py:813-874) on a harmonic
+    bowl: the returned snapshot's RMS force is the lowest one checked (the
+    force before each step); with steps that the 0.1 A cap binds, 3 steps move
+    no atom more than 0.2 A (the best snapshot is a state before a step); a
+    long stage ends near the minimum."""
+    k = 50.0
+    x0 = np.array([[3.0, -2.0, 1.0], [0.5, 0.4, -2.5]])
+    seen = []
+
+    def ff(x):
+        f = -k * x
+        seen.append(math.sqrt(float(np.sum(f * f)) / len(x)))
+        return f
+
+    xb, rms = O.fire_stage_best(ff, x0, np.full(2, 12.0), 300, dt_start=0.01, dt_max=0.1)
+    assert rms == min(seen[:-1])
+    assert np.max(np.abs(xb)) < 1e-2
+    # dt = 0.5: the uncapped first step would move atom 0 by ~3 A
+    xb3, _ = O.fire_stage_best(lambda x: -k * x, x0, np.full(2, 12.0), 3, dt_start=0.5, dt_max=0.5)
+    moved = np.linalg.norm(xb3 - x0, axis=1)
+    assert 0.0 < moved.max() <= 0.2 + 1e-12
+
+
+def test_lbfgs_reaches_known_minima():
+    """L-BFGS restatement: an anisotropic quadratic (minimum 0 at the origin) and
+    the Rosenbrock function (minimum 0 at (1, 1)); energies never rise."""
+    A = np.diag([1.0, 10.0, 100.0])
+    e = lambda x: 0.5 * float(x.reshape(-1) @ A @ x.reshape(-1))
+    g = lambda x: (A @ x.reshape(-1)).reshape(x.shape)
+    x, tr = O.lbfgs_minimize(e, g, np.array([[1.0, 1.0, 1.0]]), 200, max_stepsize=1.0)
+    assert np.max(np.abs(x)) < 1e-6 and np.all(np.diff(tr) <= 0)
+    ros = lambda x: (1 - x[0, 0]) ** 2 + 100 * (x[0, 1] - x[0, 0] ** 2) ** 2
+    rg = lambda x: np.array([[-2 * (1 - x[0, 0]) - 400 * x[0, 0] * (x[0, 1] - x[0, 0] ** 2),
+                              200 * (x[0, 1] - x[0, 0] ** 2)]])
+    x, tr = O.lbfgs_minimize(ros, rg, np.array([[-1.2, 1.0]]), 2000, max_stepsize=1.0, max_trials=40)
+    assert np.allclose(x, [[1.0, 1.0]], atol=1e-4) and np.all(np.diff(tr) <= 0)
