@@ -126,22 +126,29 @@ struct Slot {
 void torsion_entries(const prx_system_desc& d, std::vector<prx::TorsionEntry>* out, int64_t* count,
                      int a0 = 0) {
   int64_t n = 0;
+  // the nonzero terms of one torsion, paired in order: one entry per pair
   auto emit_periodic = [&](const int32_t* idx, const double* prm, int T, float kind) {
-    std::vector<const double*> terms;
-    for (int k = 0; k < T; ++k)
-      if (prm[(size_t)k * 3 + 2] != 0.0) terms.push_back(prm + (size_t)k * 3);
-    for (size_t k = 0; k < terms.size(); k += 2) {
+    const double* first = nullptr;
+    auto emit = [&](const double* t0, const double* t1) {
       ++n;
-      if (!out) continue;
+      if (!out) return;
       prx::TorsionEntry e;
       e.idx = make_int4(a0 + idx[0], a0 + idx[1], a0 + idx[2], a0 + idx[3]);
-      e.a = make_float4((float)terms[k][0], (float)terms[k][1], (float)terms[k][2], kind);
-      if (k + 1 < terms.size())
-        e.b = make_float4((float)terms[k + 1][0], (float)terms[k + 1][1], (float)terms[k + 1][2], 2.0f);
-      else
-        e.b = make_float4(0.f, 0.f, 0.f, 1.0f);
+      e.a = make_float4((float)t0[0], (float)t0[1], (float)t0[2], kind);
+      e.b = t1 ? make_float4((float)t1[0], (float)t1[1], (float)t1[2], 2.0f) : make_float4(0.f, 0.f, 0.f, 1.0f);
       out->push_back(e);
+    };
+    for (int k = 0; k < T; ++k) {
+      const double* t = prm + (size_t)k * 3;
+      if (t[2] == 0.0) continue;
+      if (first) {
+        emit(first, t);
+        first = nullptr;
+      } else {
+        first = t;
+      }
     }
+    if (first) emit(first, nullptr);
   };
   for (int t = 0; t < d.n_dihedrals; ++t)
     emit_periodic(&d.dihedral_idx[4 * t], &d.dihedral_params[(size_t)t * d.dihedral_terms * 3], d.dihedral_terms,
@@ -267,6 +274,63 @@ char* scratch_ptr() {
 }
 }  // namespace
 
+namespace {
+// Float32 patch coefficients of CMAP grids already seen: systems of one
+// force field share their maps, so an admission finds them here instead of
+// solving the splines again. Keyed by the grid's bytes (compared in full).
+struct CmapCache {
+  std::mutex mu;
+  std::vector<std::pair<std::vector<double>, std::vector<float>>> entries;  // (grid, coefficients)
+  static constexpr size_t kMaxEntries = 64;
+
+  void get(const double* grid, int G, float* out) {
+    const size_t n = (size_t)G * G;
+    {
+      std::lock_guard<std::mutex> lock(mu);
+      for (const auto& e : entries)
+        if (e.first.size() == n && std::memcmp(e.first.data(), grid, n * sizeof(double)) == 0) {
+          std::memcpy(out, e.second.data(), e.second.size() * sizeof(float));
+          return;
+        }
+    }
+    std::vector<double> c64(n * 16);
+    prx::cmap_coefficients(grid, G, c64.data());
+    std::vector<float> c32(n * 16);
+    for (size_t k = 0; k < c32.size(); ++k) c32[k] = (float)c64[k];
+    std::memcpy(out, c32.data(), c32.size() * sizeof(float));
+    std::lock_guard<std::mutex> lock(mu);
+    if (entries.size() >= kMaxEntries) entries.erase(entries.begin());
+    entries.emplace_back(std::vector<double>(grid, grid + n), std::move(c32));
+  }
+};
+CmapCache g_cmap_cache;
+
+// Pinned host memory from which a pool's admissions are copied to the device
+// asynchronously on the pool's stream. Reset only once those copies are done
+// (a synchronised stream).
+struct PinnedStage {
+  char* base = nullptr;
+  size_t cap = 0, used = 0;
+  ~PinnedStage() {
+    if (base) hipHostFree(base);
+  }
+  // Room for bytes more; false if the buffer must first grow.
+  bool fits(size_t bytes) const { return used + ((bytes + 255) & ~(size_t)255) <= cap; }
+  void grow(size_t at_least) {
+    if (base) HIPCHECK(hipHostFree(base));
+    base = nullptr;
+    cap = std::max<size_t>(at_least, std::max<size_t>(2 * cap, (size_t)4 << 20));
+    HIPCHECK(hipHostMalloc((void**)&base, cap, hipHostMallocDefault));
+    used = 0;
+  }
+  char* take(size_t bytes) {
+    char* p = base + used;
+    used += (bytes + 255) & ~(size_t)255;
+    return p;
+  }
+};
+}  // namespace
+
 struct prx_engine {
   // capacities
   int64_t cap_atoms = 0, cap_bond = 0, cap_ang = 0, cap_tors = 0, cap_cmap = 0, cap_exc = 0;
@@ -297,6 +361,10 @@ struct prx_engine {
   // pool slots admitted since the last step call: their momenta are drawn and
   // every live system's forces computed once for the whole batch (flush_pending)
   std::vector<int32_t> pending_init;
+  // pools: admission uploads go through pinned memory as asynchronous copies
+  // on the engine stream (h2d / d_fill); every reader synchronises that
+  // stream first
+  PinnedStage stage;
   int32_t* d_rec_list = nullptr;
   std::vector<Slot> slots;
   prx::DeviceView v{};
@@ -326,6 +394,7 @@ struct prx_engine {
   hipStream_t stream = nullptr;
   hipStream_t side = nullptr;  // bonded terms, forked from and joined to the pass stream
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipEvent_t ev_order = nullptr;  // orders pool uploads after the caller's stream
   bool dirty = true;            // work list / ranges need rebuilding
   bool initialized = false;
   std::vector<std::pair<int, hipGraphExec_t>> graphs;  // (steps per launch, executable)
@@ -441,6 +510,7 @@ struct prx_engine {
     HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     HIPCHECK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
     HIPCHECK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+    HIPCHECK(hipEventCreateWithFlags(&ev_order, hipEventDisableTiming));
     HIPCHECK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
   }
 
@@ -472,6 +542,7 @@ struct prx_engine {
     for (void* p : ptrs)
       if (p) hipFree(p);
     if (ev_fork) hipEventDestroy(ev_fork);
+    if (ev_order) hipEventDestroy(ev_order);
     if (ev_join) hipEventDestroy(ev_join);
     if (side) hipStreamDestroy(side);
     if (stream) hipStreamDestroy(stream);
@@ -535,9 +606,7 @@ struct prx_engine {
       s.items = std::move(np.items);
       s.nl_rows = std::move(np.rows);
       s.nl_cols = std::move(np.cols);
-      if (!np.tps.empty())
-        HIPCHECK(hipMemcpy(d_nl_tp + s.tp_b, np.tps.data(), np.tps.size() * sizeof(prx::TilePairInfo),
-                           hipMemcpyHostToDevice));
+      if (!np.tps.empty()) h2d(d_nl_tp + s.tp_b, np.tps.data(), np.tps.size() * sizeof(prx::TilePairInfo));
       prx::SlotConst& c = s.c;
       c.nl = 1;
       c.tp_base = (int32_t)s.tp_b;
@@ -547,7 +616,7 @@ struct prx_engine {
       push_const(sid, c);
       // the first evaluation builds the list
       const int32_t one = 1;
-      HIPCHECK(hipMemcpy(d_nl_flag + sid, &one, sizeof(one), hipMemcpyHostToDevice));
+      h2d(d_nl_flag + sid, &one, sizeof(one));
     } else {
       prx::build_items(s.n, s.a0, sid, plan, (int)s.xt_b, s.items);
     }
@@ -586,15 +655,15 @@ struct prx_engine {
       sqm[i] = (float)std::sqrt(m);
       mom[i].w = (float)(1.0 / m);
     }
-    HIPCHECK(hipMemcpy(v.born4 + s.a0, born4.data(), n * sizeof(float4), hipMemcpyHostToDevice));
-    HIPCHECK(hipMemcpy(v.nb4 + s.a0, nb4.data(), n * sizeof(float4), hipMemcpyHostToDevice));
-    HIPCHECK(hipMemcpy(v.fin4 + s.a0, fin4.data(), n * sizeof(double4), hipMemcpyHostToDevice));
-    HIPCHECK(hipMemcpy(v.sqrt_m + s.a0, sqm.data(), n * sizeof(float), hipMemcpyHostToDevice));
-    HIPCHECK(hipMemcpy(v.mom + s.a0, mom.data(), n * sizeof(float4), hipMemcpyHostToDevice));
-    HIPCHECK(hipMemcpy(v.slot_of + s.a0, slot_of.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
-    HIPCHECK(hipMemset(v.atom_step + s.a0, 0, n * sizeof(uint32_t)));
-    HIPCHECK(hipMemset(v.pos + s.a0, 0, n * sizeof(float4)));
-    HIPCHECK(hipMemset(v.force + s.a0, 0, n * sizeof(float4)));
+    h2d(v.born4 + s.a0, born4.data(), n * sizeof(float4));
+    h2d(v.nb4 + s.a0, nb4.data(), n * sizeof(float4));
+    h2d(v.fin4 + s.a0, fin4.data(), n * sizeof(double4));
+    h2d(v.sqrt_m + s.a0, sqm.data(), n * sizeof(float));
+    h2d(v.mom + s.a0, mom.data(), n * sizeof(float4));
+    h2d(v.slot_of + s.a0, slot_of.data(), n * sizeof(int32_t));
+    d_fill(v.atom_step + s.a0, 0, n * sizeof(uint32_t));
+    d_fill(v.pos + s.a0, 0, n * sizeof(float4));
+    d_fill(v.force + s.a0, 0, n * sizeof(float4));
 
     // bonded entries
     const int a0 = s.a0;
@@ -609,7 +678,7 @@ struct prx_engine {
         e[k].idx = make_int4(a0 + d.ub_idx[2 * b], a0 + d.ub_idx[2 * b + 1], 1, 0);
         e[k].p = make_float4((float)d.ub_params[2 * b], (float)d.ub_params[2 * b + 1], 0, 0);
       }
-      HIPCHECK(hipMemcpy(v.bonds + s.bond_b, e.data(), e.size() * sizeof(e[0]), hipMemcpyHostToDevice));
+      h2d(v.bonds + s.bond_b, e.data(), e.size() * sizeof(e[0]));
     }
     if (s.ang_n) {
       std::vector<prx::AngleEntry> e((size_t)s.ang_n);
@@ -617,30 +686,27 @@ struct prx_engine {
         e[b].idx = make_int4(a0 + d.angle_idx[3 * b], a0 + d.angle_idx[3 * b + 1], a0 + d.angle_idx[3 * b + 2], 0);
         e[b].p = make_float4((float)d.angle_params[2 * b], (float)d.angle_params[2 * b + 1], 0, 0);
       }
-      HIPCHECK(hipMemcpy(v.angles + s.ang_b, e.data(), e.size() * sizeof(e[0]), hipMemcpyHostToDevice));
+      h2d(v.angles + s.ang_b, e.data(), e.size() * sizeof(e[0]));
     }
     if (s.tors_n) {
       std::vector<prx::TorsionEntry> e;
       e.reserve((size_t)s.tors_n);
       torsion_entries(d, &e, nullptr, a0);
-      HIPCHECK(hipMemcpy(v.tors + s.tors_b, e.data(), e.size() * sizeof(e[0]), hipMemcpyHostToDevice));
+      h2d(v.tors + s.tors_b, e.data(), e.size() * sizeof(e[0]));
     }
     if (s.cmap_n) {
       const int G = d.cmap_grid;
-      std::vector<double> c64((size_t)G * G * 16);
       std::vector<float> c32((size_t)s.coeff_n);
-      for (int m = 0; m < d.n_cmap_maps; ++m) {
-        prx::cmap_coefficients(d.cmap_grids + (size_t)m * G * G, G, c64.data());
-        for (size_t k = 0; k < c64.size(); ++k) c32[(size_t)m * G * G * 16 + k] = (float)c64[k];
-      }
-      HIPCHECK(hipMemcpy(d_coeff + s.coeff_b, c32.data(), c32.size() * sizeof(float), hipMemcpyHostToDevice));
+      for (int m = 0; m < d.n_cmap_maps; ++m)
+        g_cmap_cache.get(d.cmap_grids + (size_t)m * G * G, G, c32.data() + (size_t)m * G * G * 16);
+      h2d(d_coeff + s.coeff_b, c32.data(), c32.size() * sizeof(float));
       std::vector<prx::CmapEntry> e((size_t)s.cmap_n);
       for (int k = 0; k < d.n_cmap; ++k) {
         const int32_t* t = &d.cmap_torsions[5 * k];
         e[k].idx = make_int4(a0 + t[0], a0 + t[1], a0 + t[2], a0 + t[3]);
         e[k].idx2 = make_int4(a0 + t[4], (int)(s.coeff_b + (int64_t)d.cmap_map[k] * G * G * 16), G, 0);
       }
-      HIPCHECK(hipMemcpy(v.cmaps + s.cmap_b, e.data(), e.size() * sizeof(e[0]), hipMemcpyHostToDevice));
+      h2d(v.cmaps + s.cmap_b, e.data(), e.size() * sizeof(e[0]));
     }
     if (s.exc_n) {
       std::vector<prx::ExceptionEntry> e((size_t)s.exc_n);
@@ -649,11 +715,10 @@ struct prx_engine {
         e[k].p = make_float4((float)d.exc_sigma[k], (float)d.exc_epsilon[k],
                              (float)(d.exc_chargeprod[k] * d.coulomb_constant), 0.0f);
       }
-      HIPCHECK(hipMemcpy(v.excs + s.exc_b, e.data(), e.size() * sizeof(e[0]), hipMemcpyHostToDevice));
+      h2d(v.excs + s.exc_b, e.data(), e.size() * sizeof(e[0]));
     }
     if (s.xt_n)
-      HIPCHECK(hipMemcpy(d_excl + s.xt_b * 128, plan.tile_masks.data(), plan.tile_masks.size() * 8,
-                         hipMemcpyHostToDevice));
+      h2d(d_excl + s.xt_b * 128, plan.tile_masks.data(), plan.tile_masks.size() * 8);
     upload_record_lists(d, s);
     // constants
     prx::SlotConst& c = s.c;
@@ -681,41 +746,69 @@ struct prx_engine {
   // fixed order (entry kind, then entry, then position in the entry).
   void upload_record_lists(const prx_system_desc& d, const Slot& s) {
     const int n = s.n;
-    std::vector<std::vector<int32_t>> lists(n);
-    auto add = [&](int local_atom, int64_t rec) { lists[local_atom].push_back((int32_t)rec); };
-    int64_t k = 0;
-    for (int b = 0; b < d.n_bonds; ++b, ++k)
-      for (int a = 0; a < 2; ++a) add(d.bond_idx[2 * b + a], v.rec_base[0] + (s.bond_b + k) * prx::kRecBond + a);
-    for (int b = 0; b < d.n_ub; ++b, ++k)
-      for (int a = 0; a < 2; ++a) add(d.ub_idx[2 * b + a], v.rec_base[0] + (s.bond_b + k) * prx::kRecBond + a);
-    for (int b = 0; b < d.n_angles; ++b)
-      for (int a = 0; a < 3; ++a) add(d.angle_idx[3 * b + a], v.rec_base[1] + (s.ang_b + b) * prx::kRecAngle + a);
+    // Each atom's records in the order bonds, UB, angles, torsions, CMAP,
+    // exceptions (entry order within each), laid out by a counting sort.
     std::vector<prx::TorsionEntry> te;
     torsion_entries(d, &te, nullptr, 0);
-    for (size_t t = 0; t < te.size(); ++t) {
-      const int ids[4] = {te[t].idx.x, te[t].idx.y, te[t].idx.z, te[t].idx.w};
-      for (int a = 0; a < 4; ++a) add(ids[a], v.rec_base[2] + (s.tors_b + (int64_t)t) * prx::kRecTors + a);
-    }
-    for (int c = 0; c < d.n_cmap; ++c)
-      for (int a = 0; a < 5; ++a) add(d.cmap_torsions[5 * c + a], v.rec_base[3] + (s.cmap_b + c) * prx::kRecCmap + a);
-    for (int x = 0; x < d.n_exceptions; ++x)
-      for (int a = 0; a < 2; ++a) add(d.exc_pairs[2 * x + a], v.rec_base[4] + (s.exc_b + x) * prx::kRecExc + a);
-    std::vector<int2> ar(n);
-    std::vector<int32_t> flat;
-    flat.reserve((size_t)s.rl_n);
+    auto visit = [&](auto&& add) {
+      int64_t k = 0;
+      for (int b = 0; b < d.n_bonds; ++b, ++k)
+        for (int a = 0; a < 2; ++a) add(d.bond_idx[2 * b + a], v.rec_base[0] + (s.bond_b + k) * prx::kRecBond + a);
+      for (int b = 0; b < d.n_ub; ++b, ++k)
+        for (int a = 0; a < 2; ++a) add(d.ub_idx[2 * b + a], v.rec_base[0] + (s.bond_b + k) * prx::kRecBond + a);
+      for (int b = 0; b < d.n_angles; ++b)
+        for (int a = 0; a < 3; ++a) add(d.angle_idx[3 * b + a], v.rec_base[1] + (s.ang_b + b) * prx::kRecAngle + a);
+      for (size_t t = 0; t < te.size(); ++t) {
+        const int ids[4] = {te[t].idx.x, te[t].idx.y, te[t].idx.z, te[t].idx.w};
+        for (int a = 0; a < 4; ++a) add(ids[a], v.rec_base[2] + (s.tors_b + (int64_t)t) * prx::kRecTors + a);
+      }
+      for (int c = 0; c < d.n_cmap; ++c)
+        for (int a = 0; a < 5; ++a)
+          add(d.cmap_torsions[5 * c + a], v.rec_base[3] + (s.cmap_b + c) * prx::kRecCmap + a);
+      for (int x = 0; x < d.n_exceptions; ++x)
+        for (int a = 0; a < 2; ++a) add(d.exc_pairs[2 * x + a], v.rec_base[4] + (s.exc_b + x) * prx::kRecExc + a);
+    };
+    std::vector<int2> ar(n, make_int2(0, 0));
+    visit([&](int atom, int64_t) { ar[atom].y += 1; });
+    int64_t total = 0;
     for (int i = 0; i < n; ++i) {
-      ar[i] = make_int2((int)(s.rl_b + (int64_t)flat.size()), (int)lists[i].size());
-      flat.insert(flat.end(), lists[i].begin(), lists[i].end());
+      ar[i].x = (int)(s.rl_b + total);
+      total += ar[i].y;
     }
-    if ((int64_t)flat.size() != s.rl_n) throw std::logic_error("bonded record count mismatch");
-    HIPCHECK(hipMemcpy(v.atom_rec + s.a0, ar.data(), n * sizeof(int2), hipMemcpyHostToDevice));
+    if (total != s.rl_n) throw std::logic_error("bonded record count mismatch");
+    std::vector<int32_t> flat((size_t)total);
+    std::vector<int32_t> fill(n, 0);
+    visit([&](int atom, int64_t rec) { flat[(size_t)(ar[atom].x - s.rl_b) + fill[atom]++] = (int32_t)rec; });
+    h2d(v.atom_rec + s.a0, ar.data(), n * sizeof(int2));
     if (!flat.empty())
-      HIPCHECK(hipMemcpy(d_rec_list + s.rl_b, flat.data(), flat.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+      h2d(d_rec_list + s.rl_b, flat.data(), flat.size() * sizeof(int32_t));
   }
 
-  void push_const(int32_t sid, const prx::SlotConst& c) {
-    HIPCHECK(hipMemcpy(v.slots + sid, &c, sizeof(c), hipMemcpyHostToDevice));
+  // Host-to-device copy and fill of admission data: asynchronous from pinned
+  // memory on the engine stream for pools, synchronous otherwise.
+  void h2d(void* dst, const void* src, size_t bytes) {
+    if (!bytes) return;
+    if (!pool) {
+      HIPCHECK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+      return;
+    }
+    if (!stage.fits(bytes)) {
+      HIPCHECK(hipStreamSynchronize(stream));  // the copies still reading the buffer
+      stage.used = 0;
+      if (!stage.fits(bytes)) stage.grow(bytes + 256);
+    }
+    char* p = stage.take(bytes);
+    std::memcpy(p, src, bytes);
+    HIPCHECK(hipMemcpyAsync(dst, p, bytes, hipMemcpyHostToDevice, stream));
   }
+  void d_fill(void* dst, int byte, size_t bytes) {
+    if (!bytes) return;
+    if (pool)
+      HIPCHECK(hipMemsetAsync(dst, byte, bytes, stream));
+    else
+      HIPCHECK(hipMemset(dst, byte, bytes));
+  }
+  void push_const(int32_t sid, const prx::SlotConst& c) { h2d(v.slots + sid, &c, sizeof(c)); }
 
   void set_params(int32_t sid, const prx_langevin_params& p) {
     if (!(p.dt_fs > 0.0) || p.temperature < 0.0 || p.gamma_ps < 0.0)
@@ -733,7 +826,7 @@ struct prx_engine {
     c.sqrt_kT = (float)std::sqrt(kT);
     c.caps = p.safety_caps ? 1 : 0;
     c.seed = p.seed;
-    HIPCHECK(hipMemset(v.warn + (size_t)sid * prx::kWarnStride, 0, prx::kWarnStride * sizeof(unsigned int)));
+    d_fill(v.warn + (size_t)sid * prx::kWarnStride, 0, prx::kWarnStride * sizeof(unsigned int));
     slots[sid].has_params = true;
     push_const(sid, c);
   }
@@ -742,13 +835,17 @@ struct prx_engine {
     if (sid < 0 || sid >= (int32_t)slots.size() || !slots[sid].live) throw std::invalid_argument("no such live slot");
     pending_init.erase(std::remove(pending_init.begin(), pending_init.end(), sid), pending_init.end());
     Slot& s = slots[sid];
-    HIPCHECK(hipMemset(v.slot_of + s.a0, 0xFF, s.n * sizeof(int32_t)));
-    HIPCHECK(hipMemset(v.atom_rec + s.a0, 0, s.n * sizeof(int2)));
-    free_entries(v.bonds, s.bond_b, s.bond_n);
-    free_entries(v.angles, s.ang_b, s.ang_n);
-    free_entries(v.tors, s.tors_b, s.tors_n);
-    free_entries(v.cmaps, s.cmap_b, s.cmap_n);
-    free_entries(v.excs, s.exc_b, s.exc_n);
+    // in stream order: after the work already queued that reads the system,
+    // before any later admission's uploads into the same ranges
+    d_fill(v.slot_of + s.a0, 0xFF, s.n * sizeof(int32_t));
+    d_fill(v.atom_rec + s.a0, 0, s.n * sizeof(int2));
+    // entries all-ones: idx.x = -1 marks them free, the only field the
+    // kernels read of a free entry
+    d_fill(v.bonds + s.bond_b, 0xFF, s.bond_n * sizeof(prx::BondEntry));
+    d_fill(v.angles + s.ang_b, 0xFF, s.ang_n * sizeof(prx::AngleEntry));
+    d_fill(v.tors + s.tors_b, 0xFF, s.tors_n * sizeof(prx::TorsionEntry));
+    d_fill(v.cmaps + s.cmap_b, 0xFF, s.cmap_n * sizeof(prx::CmapEntry));
+    d_fill(v.excs + s.exc_b, 0xFF, s.exc_n * sizeof(prx::ExceptionEntry));
     a_atoms.release(s.a0, s.n);
     a_bond.release(s.bond_b, s.bond_n);
     a_ang.release(s.ang_b, s.ang_n);
@@ -758,7 +855,7 @@ struct prx_engine {
     a_coeff.release(s.coeff_b, s.coeff_n);
     a_xt.release(s.xt_b, s.xt_n);
     a_rl.release(s.rl_b, s.rl_n);
-    if (s.tp_n) HIPCHECK(hipMemset(d_nl_tp + s.tp_b, 0xFF, s.tp_n * sizeof(prx::TilePairInfo)));
+    if (s.tp_n) d_fill(d_nl_tp + s.tp_b, 0xFF, s.tp_n * sizeof(prx::TilePairInfo));
     a_tp.release(s.tp_b, s.tp_n);
     a_ent.release(s.ent_b, s.ent_n);
     s = Slot();
@@ -769,6 +866,9 @@ struct prx_engine {
   // ---- work list and ranges ---------------------------------------------
   void rebuild() {
     if (!dirty) return;
+    // the work lists are rewritten below: no kernel may still read them (the
+    // allocation-free entry point has synchronised its stream already)
+    if (!arena) HIPCHECK(hipDeviceSynchronize());
     std::vector<prx::PairItem> items;
     std::vector<int32_t> ranges;
     std::vector<int32_t> map;
@@ -971,6 +1071,7 @@ struct prx_engine {
     // the slot list may still be read by an earlier momentum draw
     HIPCHECK(hipStreamSynchronize(s));
     HIPCHECK(hipStreamSynchronize(stream));
+    stage.used = 0;  // every staged upload has landed
     HIPCHECK(hipMemcpy(d_slot_list, pending_init.data(), pending_init.size() * sizeof(int32_t),
                        hipMemcpyHostToDevice));
     prx::launch_init_momenta(v, d_slot_list, (int)pending_init.size(), s);
@@ -1473,9 +1574,14 @@ int prx_pool_admit(prx_engine* pool, const prx_system_desc* desc, const float* p
                    const prx_langevin_params* params, int32_t* slot, void* stream) {
   return guard([&] {
     need(pool && pool->pool && desc && positions_host && params && slot, "prx_pool_admit: bad arguments");
+    // The uploads are asynchronous copies on the pool's stream, after the
+    // work already queued there: admission does not wait for the device.
+    // Work the caller queued on its own stream is ordered before them.
     hipStream_t s = pool->pick(stream);
-    HIPCHECK(hipStreamSynchronize(s));
-    HIPCHECK(hipStreamSynchronize(pool->stream));
+    if (s != pool->stream) {
+      HIPCHECK(hipEventRecord(pool->ev_order, s));
+      HIPCHECK(hipStreamWaitEvent(pool->stream, pool->ev_order, 0));
+    }
     const int32_t sid = pool->admit(*desc);
     try {
       pool->set_params(sid, *params);
@@ -1483,7 +1589,7 @@ int prx_pool_admit(prx_engine* pool, const prx_system_desc* desc, const float* p
       std::vector<float4> p4(sl.n);
       for (int i = 0; i < sl.n; ++i)
         p4[i] = make_float4(positions_host[3 * i], positions_host[3 * i + 1], positions_host[3 * i + 2], 0.f);
-      HIPCHECK(hipMemcpy(pool->v.pos + sl.a0, p4.data(), sl.n * sizeof(float4), hipMemcpyHostToDevice));
+      pool->h2d(pool->v.pos + sl.a0, p4.data(), sl.n * sizeof(float4));
       // momenta and forces follow at the next step call or read, once for
       // every system admitted until then
       pool->pending_init.push_back(sid);
@@ -1500,9 +1606,13 @@ int prx_pool_set_step_limit(prx_engine* pool, int32_t slot, int64_t limit, void*
     need(pool && pool->pool && slot >= 0 && slot < (int32_t)pool->slots.size() && pool->slots[slot].live,
          "prx_pool_set_step_limit: no such live slot");
     need(limit >= 0 && limit < (int64_t)UINT32_MAX, "prx_pool_set_step_limit: 0 <= limit < 2^32 - 1 required");
-    // kernels in flight read the slot table: replace the entry between launches
-    HIPCHECK(hipStreamSynchronize(pool->pick(stream)));
-    HIPCHECK(hipStreamSynchronize(pool->stream));
+    // kernels in flight read the slot table: the entry is replaced in stream
+    // order, after the work queued on the pool's and the caller's streams
+    hipStream_t s = pool->pick(stream);
+    if (s != pool->stream) {
+      HIPCHECK(hipEventRecord(pool->ev_order, s));
+      HIPCHECK(hipStreamWaitEvent(pool->stream, pool->ev_order, 0));
+    }
     prx::SlotConst& c = pool->slots[slot].c;
     c.step_limit = (uint32_t)limit;
     pool->push_const(slot, c);
@@ -1512,8 +1622,13 @@ int prx_pool_set_step_limit(prx_engine* pool, int32_t slot, int64_t limit, void*
 int prx_pool_release(prx_engine* pool, int32_t slot, void* stream) {
   return guard([&] {
     need(pool && pool->pool, "prx_pool_release: bad arguments");
-    HIPCHECK(hipStreamSynchronize(pool->pick(stream)));
-    HIPCHECK(hipStreamSynchronize(pool->stream));
+    // the release's device writes are queued on the pool's stream after the
+    // work queued there and on the caller's stream
+    hipStream_t s = pool->pick(stream);
+    if (s != pool->stream) {
+      HIPCHECK(hipEventRecord(pool->ev_order, s));
+      HIPCHECK(hipStreamWaitEvent(pool->stream, pool->ev_order, 0));
+    }
     pool->release(slot);
   });
 }

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <unordered_map>
 #include <stdexcept>
 
 namespace prx {
@@ -150,32 +151,76 @@ void build_exclusions(const prx_system_desc& d, ExclPlan& plan) {
   const int n = d.n_atoms, C = d.excl_cols;
   if (C <= 0 || d.excl_idx == nullptr) return;
   // S(i, j) from row i: sum of matching entries' scales (optimization.py:55-57).
-  std::map<std::pair<int, int>, std::pair<double, double>> row;
-  for (int i = 0; i < n; ++i)
+  // Row i's entries sorted by j (a stable insertion sort of at most C, so a
+  // j's entries are summed in column order), duplicates summed, in CSR form.
+  struct Ent {
+    int j;
+    double v, e;
+  };
+  std::vector<Ent> ent;
+  ent.reserve((size_t)n * std::min(C, 16));
+  std::vector<int64_t> off((size_t)n + 1, 0);
+  for (int i = 0; i < n; ++i) {
+    const size_t r0 = ent.size();
     for (int k = 0; k < C; ++k) {
       const int j = d.excl_idx[(size_t)i * C + k];
       if (j < 0) continue;
       if (j >= n) throw std::invalid_argument("exclusion index out of range");
       if (j == i) continue;
-      auto& e = row[{i, j}];
-      e.first += d.excl_scale_vdw ? d.excl_scale_vdw[(size_t)i * C + k] : 0.0;
-      e.second += d.excl_scale_elec ? d.excl_scale_elec[(size_t)i * C + k] : 0.0;
+      Ent x{j, d.excl_scale_vdw ? d.excl_scale_vdw[(size_t)i * C + k] : 0.0,
+            d.excl_scale_elec ? d.excl_scale_elec[(size_t)i * C + k] : 0.0};
+      size_t p = ent.size();
+      ent.push_back(x);
+      while (p > r0 && ent[p - 1].j > x.j) {
+        ent[p] = ent[p - 1];
+        --p;
+      }
+      ent[p] = x;
     }
-  // Symmetrise: the unordered pair carries (S(i,j) + S(j,i)) / 2 of each scale,
-  // which is the reference's 1/2 sum over both ordered pairs.
-  std::map<std::pair<int, int>, std::pair<double, double>> pairs;
-  for (const auto& kv : row) {
-    const int i = kv.first.first, j = kv.first.second;
-    const int a = std::min(i, j), b = std::max(i, j);
-    if (pairs.count({a, b})) continue;
-    auto it1 = row.find({a, b}), it2 = row.find({b, a});
-    const double v1 = it1 != row.end() ? it1->second.first : 1.0, e1 = it1 != row.end() ? it1->second.second : 1.0;
-    const double v2 = it2 != row.end() ? it2->second.first : 1.0, e2 = it2 != row.end() ? it2->second.second : 1.0;
-    pairs[{a, b}] = {0.5 * (v1 + v2), 0.5 * (e1 + e2)};
+    size_t m = r0;
+    for (size_t r = r0; r < ent.size(); ++r) {
+      if (m > r0 && ent[m - 1].j == ent[r].j) {
+        ent[m - 1].v += ent[r].v;
+        ent[m - 1].e += ent[r].e;
+      } else {
+        ent[m++] = ent[r];
+      }
+    }
+    ent.resize(m);
+    off[(size_t)i + 1] = (int64_t)m;
   }
-  std::map<std::pair<int, int>, int> tile_index;
-  for (const auto& kv : pairs) {
-    const double sv = kv.second.first, se = kv.second.second;
+  auto find = [&](int i, int j) -> const Ent* {
+    for (int64_t r = off[i]; r < off[(size_t)i + 1]; ++r)
+      if (ent[r].j >= j) return ent[r].j == j ? &ent[r] : nullptr;
+    return nullptr;
+  };
+  // Symmetrise: the unordered pair (a < b) carries (S(a,b) + S(b,a)) / 2 of
+  // each scale, the reference's 1/2 sum over both ordered pairs; a missing
+  // ordered pair counts as scale 1. Each unordered pair is formed once, from
+  // its (a, b) entry or, when that is absent, from its (b, a) entry.
+  struct Pair {
+    uint64_t key;  // a << 32 | b
+    double v, e;
+  };
+  std::vector<Pair> pairs;
+  pairs.reserve(ent.size());
+  for (int i = 0; i < n; ++i)
+    for (int64_t r = off[i]; r < off[(size_t)i + 1]; ++r) {
+      const int j = ent[r].j;
+      const int a = std::min(i, j), b = std::max(i, j);
+      const Ent* e1 = find(a, b);
+      if (i > j && e1) continue;  // formed from (a, b)
+      const Ent* e2 = find(b, a);
+      const double v1 = e1 ? e1->v : 1.0, s1 = e1 ? e1->e : 1.0;
+      const double v2 = e2 ? e2->v : 1.0, s2 = e2 ? e2->e : 1.0;
+      pairs.push_back({((uint64_t)a << 32) | (uint32_t)b, 0.5 * (v1 + v2), 0.5 * (s1 + s2)});
+    }
+  // classes and tiles numbered in increasing (a, b) order
+  std::sort(pairs.begin(), pairs.end(), [](const Pair& a, const Pair& b) { return a.key < b.key; });
+  const int64_t nT = (n + kTile - 1) / kTile;
+  std::unordered_map<int64_t, int> tile_index;
+  for (const Pair& x : pairs) {
+    const double sv = x.v, se = x.e;
     if (sv == 1.0 && se == 1.0) continue;
     int cls = -1;
     for (size_t c = 0; c < plan.classes.size(); ++c)
@@ -187,22 +232,17 @@ void build_exclusions(const prx_system_desc& d, ExclPlan& plan) {
       cls = (int)plan.classes.size() - 1;
     }
     const int code = cls + 1;  // 1..3
-    const int i = kv.first.first, j = kv.first.second;
+    const int i = (int)(x.key >> 32), j = (int)(uint32_t)x.key;
     const int I = i / kTile, J = j / kTile;
-    auto it = tile_index.find({I, J});
-    int t;
-    if (it == tile_index.end()) {
-      t = (int)plan.tiles.size();
-      tile_index[{I, J}] = t;
+    auto ins = tile_index.emplace((int64_t)I * nT + J, (int)plan.tiles.size());
+    if (ins.second) {
       plan.tiles.push_back({I, J});
       plan.tile_masks.resize(plan.tile_masks.size() + 128, 0ull);
-    } else {
-      t = it->second;
     }
-    unsigned long long* m = &plan.tile_masks[(size_t)t * 128];
+    unsigned long long* mk = &plan.tile_masks[(size_t)ins.first->second * 128];
     auto setbit = [&](int lane, int bit) {
-      if (code & 1) m[lane * 2 + 0] |= (1ull << bit);
-      if (code & 2) m[lane * 2 + 1] |= (1ull << bit);
+      if (code & 1) mk[lane * 2 + 0] |= (1ull << bit);
+      if (code & 2) mk[lane * 2 + 1] |= (1ull << bit);
     };
     setbit(i - I * kTile, j - J * kTile);
     if (I == J) setbit(j - J * kTile, i - I * kTile);

@@ -246,11 +246,18 @@ class GBPool:
         self._descs = {}
         self.sizes = {}
 
-    def admit(self, sysd: dict, params: Langevin, positions=None, **opts) -> int:
+    @staticmethod
+    def describe(sysd: dict, **opts):
+        """The host-side description of a system that admit takes (no GPU
+        work): made ahead, for example while the pool steps."""
+        return N.Desc(sysd, **opts)
+
+    def admit(self, sysd: dict, params: Langevin, positions=None, desc=None, **opts) -> int:
         """Upload one system; returns its slot. Its momenta and forces are
         computed at the next step call or read, together with every other
-        system admitted since the last one."""
-        d = N.Desc(sysd, **opts)
+        system admitted since the last one. desc: the system's describe()
+        result, if made ahead."""
+        d = desc if desc is not None else N.Desc(sysd, **opts)
         x = np.ascontiguousarray(np.asarray(sysd["positions"] if positions is None else positions, np.float32))
         slot = ctypes.c_int32()
         p = params.c()

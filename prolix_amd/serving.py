@@ -84,9 +84,13 @@ def serve(requests, hbm_budget_bytes: float = 200e9, max_systems: int = 1024, ch
     Each admitted system carries its step count as a device step limit
     (GBPool.set_step_limit): a chunk may run past the end of a request without
     changing its result, so chunks need not stop at the next request's end.
-    After each chunk the finished (and failed) systems are read back and
-    evicted and the arrived requests admitted; the pool computes the new
-    systems' momenta and forces together at the next chunk.
+    Admission queues its uploads behind the running chunk without waiting for
+    it, so while a chunk runs the loop admits as many arrived requests as
+    systems finish in that chunk: they join the next chunk, which they start
+    with no more than max_systems systems stepping. After each chunk the
+    finished (and failed) systems are read back and evicted and further
+    arrived requests admitted; the pool computes the new systems' momenta and
+    forces together at the start of the next chunk.
 
     chunk_steps: steps per chunk, or with chunk_s the longest chunk; chunk_s:
     the target wall time of a chunk, whose length then follows the step rate
@@ -114,16 +118,20 @@ def serve(requests, hbm_budget_bytes: float = 200e9, max_systems: int = 1024, ch
     steady = None  # (wall, system-steps, pair-steps) when the last request was admitted
     t_admit = t_launch = t_wait = t_evict = 0.0  # host wall time spent in each phase
     sample = set(sample_energies)
-    while waiting or live:
+    descs = {}  # rid -> host-side description, made while the device steps
+    def admit_arrived(limit):
+        """Admit arrived requests in arrival order, at most limit, while the
+        budget and the pool allow; returns the slots admitted."""
+        nonlocal used
+        got = []
         t = now()
-        # admit arrived requests in arrival order while the budget allows
-        ta = clock()
-        while waiting and waiting[0].arrival <= t and len(live) < max_systems:
+        while waiting and waiting[0].arrival <= t and len(got) < limit:
             r = waiting[0]
             cost = hbm_bytes(len(r.system["charges"]), _bonded(r.system))
             if used + cost > hbm_budget_bytes and live:
                 break
-            slot = pool.admit(r.system, Langevin(r.dt_fs, r.temperature, r.gamma_ps, r.seed, safety_caps))
+            slot = pool.admit(r.system, Langevin(r.dt_fs, r.temperature, r.gamma_ps, r.seed, safety_caps),
+                              desc=descs.pop(r.rid, None))
             if slot < 0:
                 if not live:
                     raise RuntimeError(f"request {r.rid} ({len(r.system['charges'])} atoms) does not fit the pool")
@@ -132,6 +140,12 @@ def serve(requests, hbm_budget_bytes: float = 200e9, max_systems: int = 1024, ch
             waiting.pop(0)
             used += cost
             live[slot] = [r, r.n_steps, now(), cost]
+            got.append(slot)
+        return got
+
+    while waiting or live:
+        ta = clock()
+        admit_arrived(max_systems - len(live))
         t_admit += clock() - ta
         if steady is None and not waiting:
             steady = (now(), sys_steps, pair_steps)
@@ -143,13 +157,26 @@ def serve(requests, hbm_budget_bytes: float = 200e9, max_systems: int = 1024, ch
         ts = clock()
         pool.steps(chunk)
         tl = clock()
+        # while the chunk runs: admit as many arrived requests as systems
+        # finish in it (they join the next chunk), then describe the requests
+        # likely admitted after it
+        stepping = set(live)
+        n_fin = sum(1 for v in live.values() if v[1] <= chunk)
+        admit_arrived(n_fin)
+        for r in waiting[:n_fin + 1]:
+            if r.rid not in descs and r.arrival <= now() + 0.5:
+                descs[r.rid] = GBPool.describe(r.system)
+        tj = clock()
         torch.cuda.synchronize()
         tw = clock()
         t_launch += tl - ts
-        t_wait += tw - tl
+        t_admit += tj - tl
+        t_wait += tw - tj
         chunks += 1
         te = clock()
-        for v in live.values():
+        for slot, v in live.items():
+            if slot not in stepping:
+                continue  # admitted during the chunk
             k = min(chunk, v[1])  # steps the system took (it stops at its limit)
             v[1] -= k
             n = len(v[0].system["charges"])
@@ -159,7 +186,7 @@ def serve(requests, hbm_budget_bytes: float = 200e9, max_systems: int = 1024, ch
         # momenta turned non-finite is evicted with an error result; the others
         # never read its atoms, so they continue unchanged
         failed = {slot for slot, c in pool.nonfinite_counts().items() if c > 0}
-        for slot in [s for s, v in live.items() if v[1] <= 0 or s in failed]:
+        for slot in [s for s, v in live.items() if s in stepping and (v[1] <= 0 or s in failed)]:
             r, _, adm, cost = live.pop(slot)
             x, p, f, steps = pool.read(slot)
             err = None
@@ -198,9 +225,10 @@ def serve(requests, hbm_budget_bytes: float = 200e9, max_systems: int = 1024, ch
                                  "aggregate_ns_per_day": st_sys * 1e-6 * 86400.0 / max(st_wall, 1e-9)
                                  * (reqs[0].dt_fs if reqs else 1.0),
                                  "pair_interactions_per_s": st_pair / max(st_wall, 1e-9)},
-        # admit: uploads; launch: momenta, forces, work lists and graphs of the
-        # admitted systems plus the enqueue of the chunk; wait: until the chunk
-        # ends; evict: counters, read-back and release
+        # admit: host work of admissions (those made while a chunk ran overlap
+        # it); launch: momenta, forces, work lists and graphs of the admitted
+        # systems plus the enqueue of the chunk; wait: until the chunk ends;
+        # evict: counters, read-back and release
         "host_time_s": {"admit": t_admit, "launch": t_launch, "wait": t_wait, "evict": t_evict},
     }
     return sorted(results, key=lambda r: r.rid), stats
