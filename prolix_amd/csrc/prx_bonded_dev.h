@@ -18,12 +18,33 @@ __device__ __forceinline__ float dot3(float3 a, float3 b) { return a.x * b.x + a
 __device__ __forceinline__ float3 cross3(float3 a, float3 b) {
   return make_float3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
+// a * b - c * d to about one rounding of the result (Kahan's difference of
+// products with fused multiply-adds: w = c d rounded, e = w - c d exactly).
+// Cross products of nearly parallel vectors keep their relative accuracy,
+// where the plain form loses ~|a||b| / |a x b| roundings.
+__device__ __forceinline__ float dop(float a, float b, float c, float d) {
+  const float w = c * d;
+  const float e = fmaf(-c, d, w);
+  const float f = fmaf(a, b, -w);
+  return f + e;
+}
+__device__ __forceinline__ float3 cross3_acc(float3 a, float3 b) {
+  return make_float3(dop(a.y, b.z, a.z, b.y), dop(a.z, b.x, a.x, b.z), dop(a.x, b.y, a.y, b.x));
+}
 __device__ __forceinline__ float3 scl3(float s, float3 a) { return make_float3(s * a.x, s * a.y, s * a.z); }
 __device__ __forceinline__ float3 add3(float3 a, float3 b) { return make_float3(a.x + b.x, a.y + b.y, a.z + b.z); }
 __device__ __forceinline__ void put(float4* r, float3 f) { *r = make_float4(f.x, f.y, f.z, 0.0f); }
 
-// Dihedral angle as bonded.py:84-107 (b1 norm + 1e-8) and its gradient with
-// respect to the four atoms (Blondel & Karplus, J. Comput. Chem. 17, 1132).
+// Dihedral angle of bonded.py:84-107 and its gradient with respect to the
+// four atoms (Blondel & Karplus, J. Comput. Chem. 17, 1132). With A = b1 x b0
+// and B = b1 x b2 (b0 = p0 - p1, b1 = p2 - p1, b2 = p3 - p2), the reference's
+// projections satisfy vv.ww = A.B / |b1|^2 and, as A x B = b1 (b1.(b0 x b2)),
+// (u x vv).ww = (A x B).b1 / |b1|^3, so phi = atan2((A x B).b1 / |b1|, A.B)
+// (its 1e-8 in the norm of b1 changes phi by far less than a rounding). A, B
+// and A x B come from cross3_acc, so phi and the gradient keep float32
+// accuracy when three of the atoms are nearly collinear (|A| or |B| much
+// smaller than the products of the bond lengths); a dot product such as b0.B
+// would not, its terms cancelling to sin(theta) of their size.
 struct Dihedral {
   float phi;
   float3 g0, g1, g2, g3;
@@ -32,14 +53,11 @@ struct Dihedral {
 __device__ __forceinline__ Dihedral dihedral(float3 p0, float3 p1, float3 p2, float3 p3) {
   const float3 b0 = sub3(p0, p1), b1 = sub3(p2, p1), b2 = sub3(p3, p2);
   const float b1n = sqrtf(dot3(b1, b1));
-  const float3 u = scl3(1.0f / (b1n + 1e-8f), b1);
-  const float3 vv = sub3(b0, scl3(dot3(b0, u), u));
-  const float3 ww = sub3(b2, scl3(dot3(b2, u), u));
-  Dihedral d;
-  d.phi = atan2f(dot3(cross3(u, vv), ww), dot3(vv, ww));
   // F = p0 - p1, G = p1 - p2, H = p3 - p2
   const float3 F = b0, G = scl3(-1.0f, b1), H = b2;
-  const float3 A = cross3(F, G), B = cross3(H, G);
+  const float3 A = cross3_acc(F, G), B = cross3_acc(H, G);
+  Dihedral d;
+  d.phi = atan2f(dot3(cross3_acc(A, B), b1) / b1n, dot3(A, B));
   const float A2 = dot3(A, A), B2 = dot3(B, B);
   const float FG = dot3(F, G), HG = dot3(H, G);
   const float ga = b1n / A2, gb = b1n / B2;
@@ -82,18 +100,27 @@ __device__ void do_angle(const DeviceView& v, int i) {
   const float Q = d1 * d2 + 1e-8f;
   const float P = dot3(v1, v2);
   const float c = P / Q;
-  const float cc = fminf(fmaxf(c, -0.999999f), 0.999999f);
-  const float th = acosf(cc);
-  const float dth = th - a.p.x;
+  // bonded.py:76-77 clips cos(theta) to +-0.999999: theta is constant beyond
+  // it and the force zero
   const bool inside = c > -0.999999f && c < 0.999999f;
-  const float dEdc = inside ? -a.p.y * dth / sqrtf(1.0f - cc * cc) : 0.0f;
-  const float k1 = P / (Q * Q);
-  const float3 g0 = sub3(scl3(1.0f / Q, v2), scl3(k1 * d2 / d1, v1));
-  const float3 g2 = sub3(scl3(1.0f / Q, v1), scl3(k1 * d1 / d2, v2));
+  // theta = acos(P / Q) of the reference (Q with its 1e-8) without acos and
+  // 1 / sqrt(1 - c^2), which lose ~1e-7 / (1 - c^2) relative near a straight
+  // angle (1e-3 at 179.5 degrees): with n = v1 x v2 from cross3_acc,
+  // Q^2 - P^2 = |n|^2 + 2e-8 |v1||v2| + 1e-16 = y^2, so theta = atan2(y, P) and
+  // dc/dx_i = -(v1 x n) / (Q |v1|^2) + 1e-8 P v1 / (|v1|^2 Q^2), likewise for
+  // x_k; the force on i is k (theta - theta0) (Q / y) dc/dx_i.
+  const float3 n = cross3_acc(v1, v2);
+  const float y = sqrtf(dot3(n, n) + 2e-8f * d1 * d2 + 1e-16f);
+  const float th = inside ? atan2f(y, P) : acosf(fminf(fmaxf(c, -0.999999f), 0.999999f));
+  const float dth = th - a.p.x;
+  const float g = inside ? a.p.y * dth * Q / y : 0.0f;
+  const float pe = P * 1e-8f / (Q * Q);
+  const float3 fi = scl3(g, add3(scl3(-1.0f / (Q * d1 * d1), cross3(v1, n)), scl3(pe / (d1 * d1), v1)));
+  const float3 fk = scl3(g, add3(scl3(1.0f / (Q * d2 * d2), cross3(v2, n)), scl3(pe / (d2 * d2), v2)));
   float4* rec = v.rec + v.rec_base[1] + (int64_t)i * kRecAngle;
-  put(rec + 0, scl3(-dEdc, g0));
-  put(rec + 1, scl3(dEdc, add3(g0, g2)));
-  put(rec + 2, scl3(-dEdc, g2));
+  put(rec + 0, fi);
+  put(rec + 1, scl3(-1.0f, add3(fi, fk)));
+  put(rec + 2, fk);
   if (ENERGY) v.e_angle[i] = 0.5 * (double)a.p.y * (double)dth * (double)dth;
 }
 

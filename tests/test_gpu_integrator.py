@@ -145,6 +145,7 @@ def test_chunking_and_batch_bitwise():
         e1.steps(100)
         xa = e1.state()[0].cpu().numpy()
         e1.close()
+        assert np.isfinite(xa).all(), k
         assert np.array_equal(xa, xb[k]), k
         assert np.array_equal(xa, xc[k]), k
 

@@ -65,6 +65,7 @@ def test_recenter_com_centres_each_system_and_keeps_momenta():
     eng.recenter_com()
     x1, p1, f1 = (a.cpu().numpy() for a in eng.state())
     eng.close()
+    assert np.isfinite(x0).all() and np.isfinite(p0).all()
     assert np.array_equal(p0, p1) and np.array_equal(f0, f1)
     for xs0, xs1, m in zip(np.split(x0, [60]), np.split(x1, [60]), masses):
         assert np.abs(_com(xs1.astype(np.float64), m)).max() < 1e-4

@@ -93,18 +93,21 @@ def test_list_systems_packed_equal_alone():
     systems = [protein(int(n), 30 + k, radius=synth.compact_radius(int(n))) for k, n in enumerate((900, 1500, 2100))]
     print("[gpu] 3 list systems packed vs alone", flush=True)
     eng = GBEngine(systems, cutoff=rc, gb_cutoff=True)
-    eng.init([s["positions"] for s in systems], [Langevin(1.0, 300.0, 1.0, seed=7 + k) for k in range(3)])
+    # the batched step's caps: the unrelaxed compact starts stay bounded
+    eng.init([s["positions"] for s in systems], [Langevin(1.0, 300.0, 1.0, seed=7 + k, safety_caps=True)
+                                                 for k in range(3)])
     eng.steps(25)
     x, p, f = (a.cpu().numpy() for a in eng.state())
     off = eng.offsets
     eng.close()
     for k, s in enumerate(systems):
         one = GBEngine(s, cutoff=rc, gb_cutoff=True)
-        one.init(s["positions"], Langevin(1.0, 300.0, 1.0, seed=7 + k))
+        one.init(s["positions"], Langevin(1.0, 300.0, 1.0, seed=7 + k, safety_caps=True))
         one.steps(25)
         xa, pa, fa = (a.cpu().numpy() for a in one.state())
         one.close()
         sl = slice(off[k], off[k + 1])
+        assert np.isfinite(xa).all() and np.isfinite(pa).all()
         assert np.array_equal(xa, x[sl]) and np.array_equal(pa, p[sl]) and np.array_equal(fa, f[sl])
 
 

@@ -165,6 +165,36 @@ def test_harmonic_impropers_exceptions_ub():
     assert info["force_rel_rms"] < F_TOL
 
 
+@pytest.mark.parametrize("theta_deg", [179.0, 179.6, 179.85])
+def test_near_straight_angle_and_collinear_torsions(theta_deg):
+    """The 8-atom chain (golden chain8) with its angle 1-2-3 bent to nearly
+    180 degrees (inside the reference's cos clip at +-0.999999) and placed
+    35 A from the origin: the angle's force and those of the two torsions
+    through the straight triple are ill-conditioned in float32 (1 / sin theta,
+    |b1 x b2| << |b1||b2|) and must still match float64 to the force bar,
+    atom by atom."""
+    from tests.golden import make_golden as G
+
+    sysd = G.chain8()
+    x = np.asarray(sysd["positions"], np.float64) + 35.0
+    u = (x[1] - x[2]) / np.linalg.norm(x[1] - x[2])
+    w = np.cross(u, [0.3, 0.4, 0.866])
+    w /= np.linalg.norm(w)
+    th = np.radians(theta_deg)
+    shift = x[2] + 1.52 * (np.cos(th) * u + np.sin(th) * w) - x[3]
+    x[[3, 4, 5, 7]] += shift  # atom 3 and the atoms behind it move together
+    sysd["positions"] = x
+    sysd = f32_round(sysd)
+    xs = sysd["positions"]
+    v1, v2 = xs[1] - xs[2], xs[3] - xs[2]
+    c = v1 @ v2 / (np.linalg.norm(v1) * np.linalg.norm(v2))
+    assert -0.999999 < c < -0.99984  # inside the clip, within 1 degree of straight
+    _, f, _, terms, fo, info = _compare(f"straight_angle_{theta_deg}", sysd)
+    assert info["total_rel"] < E_TOL
+    assert info["force_rel_rms"] < F_TOL
+    assert np.max(np.abs(f - fo)) < F_TOL * np.max(np.abs(fo))
+
+
 def test_deterministic_and_batch_equals_single():
     """Fixed-order accumulation: repeated calls and batched vs alone are bitwise equal."""
     from prolix_amd.engine import GBEngine

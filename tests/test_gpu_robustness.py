@@ -69,11 +69,15 @@ def test_apply_fn_is_a_function_of_its_state():
     fn = make_energy_fn(None, sysd, implicit_solvent=True, cutoff=0)
     m = np.asarray(sysd["masses"], np.float64) / S.PS_MASS_TO_AMU
     init_fn, apply_fn = S.nvt_langevin(fn, None, dt=1e-3, kT=S.BOLTZMANN_KCAL * 300.0, gamma=1.0, mass=m)
-    s0 = init_fn(np.array([0, 7], np.uint32), sysd["positions"])
+    # from relaxed positions: the raw synthetic start blows up within 20 steps
+    # without the batched step's caps
+    x0 = S.run_minimization(fn, sysd["positions"], sd_steps=50)
+    s0 = init_fn(np.array([0, 7], np.uint32), x0)
     s1 = apply_fn(s0, n_steps=10)
     x1 = s1.positions.cpu().numpy()
     s2 = apply_fn(s1, n_steps=10)
     x2, p2 = s2.positions.cpu().numpy(), s2.momentum.cpu().numpy()
+    assert np.isfinite(x2).all() and np.isfinite(p2).all()
     fn(sysd["positions"] + 0.1)  # an energy call at other positions
     s1b = apply_fn(s0, n_steps=10)  # an older state: replayed, not continued
     assert s1b.step == 10 and np.array_equal(s1b.positions.cpu().numpy(), x1)
