@@ -44,57 +44,120 @@ __device__ double block_sum(double x, double* sh) {
 // tors_b, tors_e, cmap_b, cmap_e, exc_b, exc_e.
 constexpr int kRangeInts = 14;
 
-__global__ __launch_bounds__(kRedThreads) void k_energy_reduce(DeviceView v, const int32_t* ranges,
-                                                               double* out) {
-  __shared__ double sh[kRedThreads];
+// Energy reduction: one 1024-thread block per listed system. Each thread sums
+// its strided share of every term in index order (the work-item loop issues
+// four iterations' loads before their adds: the loop is bound by load
+// latency, not bandwidth), then the lanes of each wave are combined by a
+// fixed butterfly and the 16 waves in wave order, so the result is
+// deterministic.
+constexpr int kEnThreads = 1024;
+constexpr int kEnWaves = kEnThreads / 64;
+
+// Elements b + t, b + t + T, ... of [b, e) (T = kEnThreads) in that order:
+// acc(kind(k), energy(k)), with four elements' loads issued before their adds.
+template <class K, class E, class A>
+__device__ __forceinline__ void strided4(int b, int e, int t, K kind, E energy, A acc) {
+  int k = b + t;
+  for (; k + 3 * kEnThreads < e; k += 4 * kEnThreads) {
+    int z[4];
+    double x[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      z[u] = kind(k + u * kEnThreads);
+      x[u] = energy(k + u * kEnThreads);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc(z[u], x[u]);
+  }
+  for (; k < e; k += kEnThreads) acc(kind(k), energy(k));
+}
+
+__device__ __forceinline__ double wave_sum_lane0(double x) {
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+  return x;  // lane 0's value is used
+}
+
+__global__ __launch_bounds__(kEnThreads) void k_energy_reduce(DeviceView v, const int32_t* ranges,
+                                                              double* out) {
+  constexpr int kTerms = 12;
+  __shared__ double part[kTerms][kEnWaves];
   const int32_t* R = ranges + blockIdx.x * kRangeInts;
   const int t = threadIdx.x;
   double e_gbp = 0, e_cl = 0, e_lj = 0;
-  for (int i = R[0] + t; i < R[1]; i += kRedThreads) {
+  int i = R[0] + t;
+  for (; i + 3 * kEnThreads < R[1]; i += 4 * kEnThreads) {
+    double a[4][3];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) a[u][c] = v.e_items[(size_t)(i + u * kEnThreads) * 3 + c];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      e_gbp += a[u][0];
+      e_cl += a[u][1];
+      e_lj += a[u][2];
+    }
+  }
+  for (; i < R[1]; i += kEnThreads) {
     e_gbp += v.e_items[(size_t)i * 3 + 0];
     e_cl += v.e_items[(size_t)i * 3 + 1];
     e_lj += v.e_items[(size_t)i * 3 + 2];
   }
   double e_self = 0, e_ace = 0;
-  for (int i = t; i < R[3]; i += kRedThreads) {
-    e_self += v.e_atom[(size_t)(R[2] + i) * 2 + 0];
-    e_ace += v.e_atom[(size_t)(R[2] + i) * 2 + 1];
+  for (int k = t; k < R[3]; k += kEnThreads) {
+    e_self += v.e_atom[(size_t)(R[2] + k) * 2 + 0];
+    e_ace += v.e_atom[(size_t)(R[2] + k) * 2 + 1];
   }
+  // A system's bonded ranges hold only its own entries (free entries lie
+  // outside every listed range), so no entry is tested for being free; the
+  // kind of a bond (bond / Urey-Bradley) and of a torsion entry (proper /
+  // improper) is read beside its energy. Four entries' loads per iteration.
   double e_bond = 0, e_ub = 0;
-  for (int i = R[4] + t; i < R[5]; i += kRedThreads) {
-    if (v.bonds[i].idx.x < 0) continue;
-    if (v.bonds[i].idx.z == 0) e_bond += v.e_bond[i]; else e_ub += v.e_bond[i];
-  }
+  strided4(R[4], R[5], t, [&](int k) { return v.bonds[k].idx.z; }, [&](int k) { return v.e_bond[k]; },
+           [&](int z, double e) { if (z == 0) e_bond += e; else e_ub += e; });
   double e_ang = 0;
-  for (int i = R[6] + t; i < R[7]; i += kRedThreads)
-    if (v.angles[i].idx.x >= 0) e_ang += v.e_angle[i];
+  strided4(R[6], R[7], t, [&](int) { return 0; }, [&](int k) { return v.e_angle[k]; },
+           [&](int, double e) { e_ang += e; });
   double e_dih = 0, e_imp = 0;
-  for (int i = R[8] + t; i < R[9]; i += kRedThreads) {
-    if (v.tors[i].idx.x < 0) continue;
-    if (v.tors[i].a.w == 0.0f) e_dih += v.e_tors[i]; else e_imp += v.e_tors[i];
-  }
+  strided4(R[8], R[9], t, [&](int k) { return v.tors[k].a.w == 0.0f ? 0 : 1; }, [&](int k) { return v.e_tors[k]; },
+           [&](int z, double e) { if (z == 0) e_dih += e; else e_imp += e; });
   double e_cm = 0;
-  for (int i = R[10] + t; i < R[11]; i += kRedThreads)
-    if (v.cmaps[i].idx.x >= 0) e_cm += v.e_cmap[i];
+  strided4(R[10], R[11], t, [&](int) { return 0; }, [&](int k) { return v.e_cmap[k]; },
+           [&](int, double e) { e_cm += e; });
   double e_ex = 0;
-  for (int i = R[12] + t; i < R[13]; i += kRedThreads)
-    if (v.excs[i].idx.x >= 0) e_ex += v.e_exc[i];
+  strided4(R[12], R[13], t, [&](int) { return 0; }, [&](int k) { return v.e_exc[k]; },
+           [&](int, double e) { e_ex += e; });
+  const double mine[kTerms] = {e_bond, e_ang, e_ub, e_dih, e_imp, e_cm, e_lj, e_ex, e_gbp, e_self, e_cl, e_ace};
+  const int lane = t & 63, wave = t >> 6;
+#pragma unroll
+  for (int k = 0; k < kTerms; ++k) {
+    const double w = wave_sum_lane0(mine[k]);
+    if (lane == 0) part[k][wave] = w;
+  }
+  __syncthreads();
+  if (t != 0) return;
+  double sum[kTerms];
+  for (int k = 0; k < kTerms; ++k) {
+    double acc = 0.0;
+    for (int w = 0; w < kEnWaves; ++w) acc += part[k][w];
+    sum[k] = acc;
+  }
   double terms[PRX_N_ENERGY];
-  terms[PRX_E_BOND] = block_sum(e_bond, sh);
-  terms[PRX_E_ANGLE] = block_sum(e_ang, sh);
-  terms[PRX_E_UREY_BRADLEY] = block_sum(e_ub, sh);
-  terms[PRX_E_DIHEDRAL] = block_sum(e_dih, sh);
-  terms[PRX_E_IMPROPER] = block_sum(e_imp, sh);
-  terms[PRX_E_CMAP] = block_sum(e_cm, sh);
-  terms[PRX_E_LJ] = block_sum(e_lj, sh);
-  terms[PRX_E_EXCEPTION] = block_sum(e_ex, sh);
-  terms[PRX_E_GB] = block_sum(e_gbp, sh) + block_sum(e_self, sh);
-  terms[PRX_E_COULOMB] = block_sum(e_cl, sh);
-  terms[PRX_E_ACE] = block_sum(e_ace, sh);
+  terms[PRX_E_BOND] = sum[0];
+  terms[PRX_E_ANGLE] = sum[1];
+  terms[PRX_E_UREY_BRADLEY] = sum[2];
+  terms[PRX_E_DIHEDRAL] = sum[3];
+  terms[PRX_E_IMPROPER] = sum[4];
+  terms[PRX_E_CMAP] = sum[5];
+  terms[PRX_E_LJ] = sum[6];
+  terms[PRX_E_EXCEPTION] = sum[7];
+  terms[PRX_E_GB] = sum[8] + sum[9];
+  terms[PRX_E_COULOMB] = sum[10];
+  terms[PRX_E_ACE] = sum[11];
   double tot = 0.0;
   for (int k = 0; k < PRX_E_TOTAL; ++k) tot += terms[k];
   terms[PRX_E_TOTAL] = tot;
-  if (t < PRX_N_ENERGY) out[(size_t)blockIdx.x * PRX_N_ENERGY + t] = terms[t];
+  for (int k = 0; k < PRX_N_ENERGY; ++k) out[(size_t)blockIdx.x * PRX_N_ENERGY + k] = terms[k];
 }
 
 __global__ __launch_bounds__(kRedThreads) void k_kinetic(DeviceView v, const int32_t* ranges,
@@ -231,7 +294,7 @@ void launch_step_begin(const DeviceView& v, hipStream_t s) {
 }
 void launch_energy_reduce(const DeviceView& v, const int32_t* ranges, int n_slots, double* out,
                           hipStream_t s) {
-  if (n_slots) hipLaunchKernelGGL(k_energy_reduce, dim3(n_slots), dim3(kRedThreads), 0, s, v, ranges, out);
+  if (n_slots) hipLaunchKernelGGL(k_energy_reduce, dim3(n_slots), dim3(kEnThreads), 0, s, v, ranges, out);
 }
 void launch_kinetic(const DeviceView& v, const int32_t* ranges, int n_slots, double* out, hipStream_t s) {
   if (n_slots) hipLaunchKernelGGL(k_kinetic, dim3(n_slots), dim3(kRedThreads), 0, s, v, ranges, out);

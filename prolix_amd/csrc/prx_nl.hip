@@ -12,7 +12,7 @@
 // be in range: against whole tile pairs selected by box distance, this cuts
 // the pair work of a 20,000-atom globule at a 20 A cutoff by ~40%.
 //
-// Three kernels, no atomics, so every list is a fixed function of the
+// Four kernels, no atomics, so every list is a fixed function of the
 // positions:
 //   k_nl_check  per system: build this evaluation if forced, if skin = 0, or
 //               if an atom moved more than skin / 2 since the last build
@@ -20,8 +20,11 @@
 //   k_nl_mask   per tile pair (I, J > I) without exclusions: a 64-bit mask of
 //               the atoms of J within cutoff + skin of tile I's box;
 //   k_nl_rows   per tile I: prefix counts over its row's masks, the list
-//               entries, the list length, and (with a skin) the reference
-//               positions of the tile's atoms.
+//               length, and (with a skin) the reference positions of the
+//               tile's atoms;
+//   k_nl_fill   per tile pair: its atoms' entries at the row's base plus the
+//               pair's prefix (one wave per pair, so a long row is not one
+//               wave's serial loop).
 // Tile pairs that hold exclusions stay ordinary items (their masks are 0).
 #include "prx_nl_dev.h"
 
@@ -50,6 +53,12 @@ __global__ __launch_bounds__(kNlThreads) void k_nl_rows(DeviceView v) {
   const int t = wave_index();
   if (t >= v.n_tiles) return;
   nl_rows_wave(v, t);
+}
+
+__global__ __launch_bounds__(kNlThreads) void k_nl_fill(DeviceView v) {
+  const int w = wave_index();
+  if (w >= v.nl_tp_hw) return;
+  nl_fill_wave(v, w);
 }
 
 // ---- dense atom lists (the drop-in neighbour-list object) -----------------
@@ -137,6 +146,8 @@ void launch_nl_build(const DeviceView& v, const int32_t* slot_ranges, int n_slot
   if (v.nl_tp_hw > 0)
     hipLaunchKernelGGL(k_nl_mask, dim3((v.nl_tp_hw + wpb - 1) / wpb), dim3(kNlThreads), 0, s, v);
   if (v.n_tiles > 0) hipLaunchKernelGGL(k_nl_rows, dim3((v.n_tiles + wpb - 1) / wpb), dim3(kNlThreads), 0, s, v);
+  if (v.nl_tp_hw > 0)
+    hipLaunchKernelGGL(k_nl_fill, dim3((v.nl_tp_hw + wpb - 1) / wpb), dim3(kNlThreads), 0, s, v);
 }
 
 }  // namespace prx

@@ -28,12 +28,24 @@ __device__ __forceinline__ void nl_check_block(const DeviceView& v, const int32_
     return;
   }
   float m = 0.0f;
-  for (int i = threadIdx.x; i < n; i += kNlThreads) {
-    const float4 x = v.pos[a0 + i], r = v.nl_xref[a0 + i];
+  auto take = [&](const float4& x, const float4& r) {
     const float dx = x.x - r.x, dy = x.y - r.y, dz = x.z - r.z;
     const float d2 = dx * dx + dy * dy + dz * dz;
     m = (d2 > m || d2 != d2) ? (d2 != d2 ? INFINITY : d2) : m;  // a non-finite position forces a build
+  };
+  // four atoms' loads in flight per thread: the scan is bound by load latency
+  int i = threadIdx.x;
+  for (; i + 3 * kNlThreads < n; i += 4 * kNlThreads) {
+    float4 x[4], r[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      x[u] = v.pos[a0 + i + u * kNlThreads];
+      r[u] = v.nl_xref[a0 + i + u * kNlThreads];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) take(x[u], r[u]);
   }
+  for (; i < n; i += kNlThreads) take(v.pos[a0 + i], v.nl_xref[a0 + i]);
   sh[threadIdx.x] = m;
   __syncthreads();
   for (int off = kNlThreads / 2; off > 0; off >>= 1) {
@@ -77,7 +89,9 @@ __device__ __forceinline__ void nl_mask_wave(const DeviceView& v, int w) {
   if (lane == 0) v.nl_mask[w] = m;
 }
 
-// Tile t's row, by one wave.
+// Tile t's row, by one wave: the exclusive prefix of the entry counts of its
+// tile pairs (I, J > I) in J order, the row's total, and for a list with a
+// skin the reference positions of the tile's atoms.
 __device__ __forceinline__ void nl_rows_wave(const DeviceView& v, int t) {
   const int4 tl = v.tiles[t];
   const int slot = tl.z;
@@ -86,34 +100,38 @@ __device__ __forceinline__ void nl_rows_wave(const DeviceView& v, int t) {
   const int lane = threadIdx.x & 63;
   const int I = tl.w;
   const int nT = (S.n + kTile - 1) / kTile;
-  const int E = v.nl_row[t].y;
   const int tp0 = S.tp_base + I * (nT - 1) - (I * (I - 1)) / 2;  // record of (I, I + 1)
   int running = 0;
   for (int J0 = I + 1; J0 < nT; J0 += 64) {
     const int J = J0 + lane;
     const bool valid = J < nT;
-    const unsigned long long m = valid ? v.nl_mask[tp0 + (J - I - 1)] : 0ull;
-    const int c = __popcll(m);
+    const int c = valid ? __popcll(v.nl_mask[tp0 + (J - I - 1)]) : 0;
     int incl = c;  // inclusive prefix over the lanes, in lane order
     for (int off = 1; off < 64; off <<= 1) {
       const int y = __shfl_up(incl, off);
       if (lane >= off) incl += y;
     }
-    const int excl = incl - c;
-    if (valid) v.nl_pref[tp0 + (J - I - 1)] = running + excl;
-    const int n_here = min(64, nT - J0);
-    for (int jj = 0; jj < n_here; ++jj) {
-      const unsigned long long mj = ((unsigned long long)(uint32_t)__shfl((int)(uint32_t)(m >> 32), jj) << 32) |
-                                    (unsigned long long)(uint32_t)__shfl((int)(uint32_t)m, jj);
-      if (mj == 0ull) continue;  // wave-uniform
-      const int base = running + __shfl(excl, jj);
-      if ((mj >> lane) & 1ull)
-        v.nl_entries[E + base + __popcll(mj & nl_below(lane))] = S.a0 + (J0 + jj) * kTile + lane;
-    }
-    running += __shfl(incl, 63);
+    if (valid) v.nl_pref[tp0 + (J - I - 1)] = running + incl - c;
+    running += __builtin_amdgcn_readlane(incl, 63);
   }
   if (lane == 0) v.nl_count[t] = running;
   if (S.skin_half2 > 0.0f && lane < tl.y) v.nl_xref[tl.x + lane] = v.pos[tl.x + lane];
+}
+
+// Tile-pair record w, by one wave: the atoms of J in I's list, written at
+// I's entry base + the pair's prefix, in lane order -- so each row's entries
+// run in increasing atom index.
+__device__ __forceinline__ void nl_fill_wave(const DeviceView& v, int w) {
+  const TilePairInfo tp = v.nl_tp[w];
+  if (tp.ij < 0 || tp.slot < 0) return;  // a free record, or one an ordinary item covers
+  if (!v.nl_flag[tp.slot]) return;
+  const unsigned long long m = v.nl_mask[w];
+  if (m == 0ull) return;
+  const SlotConst& S = v.slots[tp.slot];
+  const int lane = threadIdx.x & 63;
+  const int I = tp.ij & 0xFFFF, J = tp.ij >> 16;
+  const int E = v.nl_row[S.tile0 + I].y + v.nl_pref[w];
+  if ((m >> lane) & 1ull) v.nl_entries[E + __popcll(m & nl_below(lane))] = S.a0 + J * kTile + lane;
 }
 
 }  // namespace prx

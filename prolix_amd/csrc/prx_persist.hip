@@ -94,6 +94,8 @@ __global__ __launch_bounds__(kPersistThreads) void k_persist_steps(DeviceView v,
       if (!grid_sync(bar, gen)) return;
       for (int t = wave; t < v.n_tiles; t += n_waves) nl_rows_wave(v, t);
       if (!grid_sync(bar, gen)) return;
+      for (int w = wave; w < v.nl_tp_hw; w += n_waves) nl_fill_wave(v, w);
+      if (!grid_sync(bar, gen)) return;
     }
     for (int w = wave; w < v.n_items; w += n_waves) born_item(v, w, L.born);
     for (int e = tid; e < n_bonded; e += n_threads) bonded_entry<false>(v, e);
