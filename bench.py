@@ -379,6 +379,7 @@ def main():
                    "failed_requests": st["failed"],
                    "accuracy": None}
         serving["host_time_s"] = st["host_time_s"]
+        serving["device_bytes"] = st["device_bytes"]
         log(f"serving: {len(reqs)} requests in {time.time() - t1:.1f}s, p50 {st['latency_p50_s']:.2f}s "
             f"p99 {st['latency_p99_s']:.2f}s, host {json.dumps({k: round(v, 2) for k, v in st['host_time_s'].items()})}")
 
@@ -395,7 +396,8 @@ def main():
                      "latency_p50_s": st["latency_p50_s"], "latency_p99_s": st["latency_p99_s"],
                      "offered_pair_steps_per_s": offered, "achieved_pair_steps_per_s": st["pair_interactions_per_s"],
                      "aggregate_ns_per_day": st["aggregate_ns_per_day"], "wall_s": st["wall_s"],
-                     "chunks": st["chunks"], "failed_requests": st["failed"], "host_time_s": st["host_time_s"]}
+                     "chunks": st["chunks"], "failed_requests": st["failed"], "host_time_s": st["host_time_s"],
+                     "device_bytes": st["device_bytes"]}
         log(f"sustained serving: offered {offered:.3e} pair-steps/s, achieved {st['pair_interactions_per_s']:.3e}, "
             f"p50 {st['latency_p50_s']:.3f}s p99 {st['latency_p99_s']:.3f}s, "
             f"host {json.dumps({k: round(v, 2) for k, v in st['host_time_s'].items()})}")

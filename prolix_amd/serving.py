@@ -103,7 +103,14 @@ def serve(requests, hbm_budget_bytes: float = 200e9, max_systems: int = 1024, ch
         raise ValueError("every request needs n_steps >= 1")
     if max_atoms is None:
         max_atoms = max(1, int(min(sum(len(r.system["charges"]) for r in reqs), 8_000_000)))
+    # device memory, measured: the pool's arenas at creation, then the peak
+    # over the run (the work lists and partial sums grow with the live set)
+    torch.cuda.synchronize()
+    free0 = torch.cuda.mem_get_info()[0]
     pool = GBPool(max_atoms=max_atoms, max_systems=max_systems)
+    torch.cuda.synchronize()
+    pool_bytes = free0 - torch.cuda.mem_get_info()[0]
+    peak_bytes = pool_bytes
     clock = clock or time.perf_counter
     t0 = clock()
     now = lambda: clock() - t0
@@ -111,6 +118,7 @@ def serve(requests, hbm_budget_bytes: float = 200e9, max_systems: int = 1024, ch
     live = {}  # slot -> [request, remaining, admitted time, footprint]
     results = []
     used = 0
+    peak_used = 0
     pair_steps = 0.0
     sys_steps = 0
     chunks = 0
@@ -122,7 +130,7 @@ def serve(requests, hbm_budget_bytes: float = 200e9, max_systems: int = 1024, ch
     def admit_arrived(limit):
         """Admit arrived requests in arrival order, at most limit, while the
         budget and the pool allow; returns the slots admitted."""
-        nonlocal used
+        nonlocal used, peak_used
         got = []
         t = now()
         while waiting and waiting[0].arrival <= t and len(got) < limit:
@@ -139,6 +147,7 @@ def serve(requests, hbm_budget_bytes: float = 200e9, max_systems: int = 1024, ch
             pool.set_step_limit(slot, r.n_steps)
             waiting.pop(0)
             used += cost
+            peak_used = max(peak_used, used)
             live[slot] = [r, r.n_steps, now(), cost]
             got.append(slot)
         return got
@@ -169,6 +178,7 @@ def serve(requests, hbm_budget_bytes: float = 200e9, max_systems: int = 1024, ch
         tj = clock()
         torch.cuda.synchronize()
         tw = clock()
+        peak_bytes = max(peak_bytes, free0 - torch.cuda.mem_get_info()[0])
         t_launch += tl - ts
         t_admit += tj - tl
         t_wait += tw - tj
@@ -230,6 +240,11 @@ def serve(requests, hbm_budget_bytes: float = 200e9, max_systems: int = 1024, ch
         # systems plus the enqueue of the chunk; wait: until the chunk ends;
         # evict: counters, read-back and release
         "host_time_s": {"admit": t_admit, "launch": t_launch, "wait": t_wait, "evict": t_evict},
+        # measured device bytes: the pool's arenas (sized by max_atoms at
+        # creation) and the peak with the work lists of the live systems; the
+        # budget admits requests by their estimated work-list footprint
+        "device_bytes": {"budget": float(hbm_budget_bytes), "pool_at_creation": int(pool_bytes),
+                         "peak": int(peak_bytes), "estimated_peak_admitted": int(peak_used)},
     }
     return sorted(results, key=lambda r: r.rid), stats
 

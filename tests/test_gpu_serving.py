@@ -200,8 +200,14 @@ def test_hbm_budget_admission_queues_and_keeps_results():
     fit = max(k for k in range(1, len(costs) + 1) if sum(sorted(costs)[:k]) <= budget)
     assert fit < len(reqs)
     print("[gpu] serve with a binding HBM budget", flush=True)
-    tight, _ = serve(reqs, hbm_budget_bytes=budget, chunk_steps=40)
+    tight, st = serve(reqs, hbm_budget_bytes=budget, chunk_steps=40)
     free, _ = serve(reqs, hbm_budget_bytes=1e15, chunk_steps=40)
+    # measured, not only modelled: the pool's arenas and the peak device use
+    db = st["device_bytes"]
+    print(f"[serve] device bytes: pool {db['pool_at_creation']}, peak {db['peak']}, budget {db['budget']:.0f}, "
+          f"estimated admitted peak {db['estimated_peak_admitted']}", flush=True)
+    assert 0 < db["pool_at_creation"] <= db["peak"]
+    assert db["estimated_peak_admitted"] <= budget
     assert any(r.admitted - r.arrival > 1e-3 for r in tight)
     events = sorted([(r.admitted, 1) for r in tight] + [(r.finished, -1) for r in tight])
     live = peak = 0
