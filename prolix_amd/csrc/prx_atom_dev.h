@@ -287,9 +287,21 @@ __device__ __forceinline__ double4 combine4(double4 x, const TileCtx& c) {
   return r;
 }
 
+// A tile of a system at its step limit, in a step graph
+// (DeviceView::skip_done): its pair items were skipped, so its gathers are
+// too, and its Born radii, pair force and stored force stay those of its
+// final positions. Uniform over the block.
+__device__ __forceinline__ bool tile_done(const DeviceView& v, int t) {
+  if (!v.skip_done) return false;
+  const int4 tl = v.tiles[t];
+  const SlotConst& S = v.slots[tl.z];
+  return S.step_limit != 0u && slot_done(S, v.atom_step[tl.x]);
+}
+
 // B_i from I_i in FP64 (generalized_born.py:81-93); dB/dI for the chain rule.
 // Neighbour-list GB systems floor 1/B at 1e-4 (generalized_born.py:333-337).
 __device__ __forceinline__ void born_gather_tile(const DeviceView& v, int t) {
+  if (tile_done(v, t)) return;
   const TileCtx c = tile_ctx(v, t);
   const double I = combine1(gather1(v, c), c);
   if (c.q != 0 || !c.valid) return;
@@ -308,6 +320,7 @@ __device__ __forceinline__ void born_gather_tile(const DeviceView& v, int t) {
 // Pair force and dE/dB; c_i = dE/dB_i * dB_i/dI_i with the GB self term and ACE (FP64).
 template <bool ENERGY>
 __device__ __forceinline__ void force_gather_tile(const DeviceView& v, int t) {
+  if (tile_done(v, t)) return;
   const TileCtx c = tile_ctx(v, t);
   const double4 a = combine4(gather4(v, c), c);
   if (c.q != 0 || !c.valid) return;
@@ -447,6 +460,7 @@ __device__ __forceinline__ float4 finish_atom(const DeviceView& v, const TileCtx
 // for that step (what k_tile_boxes computes), so the step graph needs no box
 // kernel after its first step.
 __device__ __forceinline__ void chain_gather_tile(const DeviceView& v, int t, int mode) {
+  if (tile_done(v, t)) return;
   const TileCtx c = tile_ctx(v, t);
   double4 part = gather3(v, c);
   float3 fb = c.valid ? bonded_force(v, c.g, c.q) : make_float3(0.f, 0.f, 0.f);

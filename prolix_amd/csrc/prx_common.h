@@ -182,6 +182,11 @@ struct DeviceView {
   const int32_t* nl_colitems;    // item indices
   float4* nl_xref;               // per atom: position at the last list build
   int32_t* nl_flag;              // per slot: 1 = build the list at this evaluation
+  // 1 in the captured step graphs of a pool: the pair passes and tile
+  // gathers skip the systems that have reached their step limit (their
+  // positions no longer change, so their Born radii, pair forces and stored
+  // forces stand); 0 for evaluations, which compute every item
+  int32_t skip_done;
 };
 
 // Grid-wide barrier state of the persistent step kernel (device memory).

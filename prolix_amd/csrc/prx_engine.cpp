@@ -982,6 +982,15 @@ struct prx_engine {
   int32_t total_atoms() const { return offsets.back(); }
   bool fork_bonded() const { return PRX_FORK_BONDED && total_atoms() >= fork_min_atoms; }
 
+  // The view of the captured steps: systems at their step limit skip the
+  // pair passes and the tile gathers, keeping the forces of their final
+  // positions. Only pools set step limits.
+  prx::DeviceView step_view() const {
+    prx::DeviceView g = v;
+    g.skip_done = pool ? 1 : 0;
+    return g;
+  }
+
   // The engine's view with the evaluation buffers in place of the
   // integrator's positions and forces (prx_energy_forces).
   prx::DeviceView eval_view() const {
@@ -1044,10 +1053,11 @@ struct prx_engine {
     HIPCHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
     try {
       prx::launch_step_begin(v, stream);
+      const prx::DeviceView sv = step_view();
       for (int k = 0; k < n_steps; ++k) {
         // after the first step, the previous chain gather wrote the boxes
-        force_passes(v, false, stream, k == 0);
-        prx::launch_chain_gather(v, k + 1 == n_steps ? 2 : 1, stream);
+        force_passes(sv, false, stream, k == 0);
+        prx::launch_chain_gather(sv, k + 1 == n_steps ? 2 : 1, stream);
       }
       HIPCHECK(hipGetLastError());
     } catch (...) {

@@ -399,12 +399,18 @@ __device__ __forceinline__ int classify_chunk(const DeviceView& v, const PairIte
   return kClsStraddle | (dmin2 > thr * thr * 1.0001f ? kClsObcFar : 0);
 }
 
+// A system at its step limit, in a step graph (DeviceView::skip_done).
+__device__ __forceinline__ bool skip_item(const DeviceView& v, const SlotConst& S, const PairItem& it) {
+  return v.skip_done && S.step_limit != 0u && slot_done(S, v.atom_step[it.i0]);
+}
+
 // The Born pass of work item w, by one wave.
 __device__ __forceinline__ void born_item(const DeviceView& v, int w, BornLds& L) {
   const PairItem it = v.items[w];
   const int lane = threadIdx.x & 63;
   const int fl = item_flags(it.meta);
   const SlotConst& S = v.slots[item_slot(it.meta)];
+  if (skip_item(v, S, it)) return;
   const JSrc js = j_source(v, it, lane);
   if (item_is_chunk(it)) {  // neighbour-list chunk: always the masked path with the cutoff test
     if (js.nj == 0) {
@@ -602,6 +608,7 @@ __device__ __forceinline__ void force_item(const DeviceView& v, int w, PairLds& 
   const int lane = threadIdx.x & 63;
   const int fl = item_flags(it.meta);
   const SlotConst& S = v.slots[item_slot(it.meta)];
+  if (!ENERGY && skip_item(v, S, it)) return;
   const int cls = S.cutoff2 > 0.0f ? (v.item_cls[w] & kClsCutMask) : kClsNear;
   if (cls == kClsFar) {
     if (S.gb_cut) {
@@ -726,6 +733,7 @@ __device__ __forceinline__ void chain_item(const DeviceView& v, int w, PairLds& 
   const int lane = threadIdx.x & 63;
   const int fl = item_flags(it.meta);
   const SlotConst& S = v.slots[item_slot(it.meta)];
+  if (skip_item(v, S, it)) return;
   const int cls = v.item_cls[w];
   float cut2 = 0.0f;
   if (S.gb_cut) {
