@@ -504,7 +504,8 @@ def main():
         "large_cutoff": large_out,
         "single_systems": single_out or None,
         "temperature_K": {"mean": float(np.mean(temp)), "median": float(np.median(temp)), "min": float(np.min(temp)),
-                          "max": float(np.max(temp)), "systems_above_1000K": int(np.sum(temp > 1000.0))},
+                          "max": float(np.max(temp)), "systems_above_1000K": int(np.sum(temp > 1000.0)),
+                          "median_over_target": float(np.median(temp / temps))},
         "safety_cap_events": {"vlimit": int(warn[0]), "force_cap": int(warn[1]), "dx_cap": int(warn[3])},
     }
     line = json.dumps(out)
