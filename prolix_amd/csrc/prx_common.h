@@ -43,6 +43,9 @@ constexpr int kClsCutMask = 3;
 // (r - s_j > rho_i, s_j <= r/2 and the same with i, j swapped), so the Born
 // and chain passes use the far-field polynomials without a per-pair test.
 constexpr int kClsObcFar = 4;
+// Bit 3: in addition, every pair has s <= r/4 in both directions, where the
+// far-field polynomials of degree 4 are as accurate as the wider fits.
+constexpr int kClsObcFar4 = 8;
 
 __host__ __device__ inline int32_t item_meta(int ni, int nj, int flags, int slot) {
   return ni | (nj << 7) | (flags << 14) | (slot << 16);

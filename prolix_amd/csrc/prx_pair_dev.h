@@ -224,20 +224,38 @@ __device__ __forceinline__ bool obc_far(float r, float rho, float s) {
 //   I = (s^3/2) u^2 h(s^2 u),   dI/dr = -2 (s^3/2) u^2 r^-1 k(s^2 u),
 // so each direction costs one product for the argument, the Horner chain and
 // two products; u^2 and u^2/r are shared by both directions of a pair.
+// F4: the degree-4 fits on y <= 1/16 (s <= r/4), for items whose every pair
+// is that far apart (kClsObcFar4); else the fits on y <= 1/4.
+template <bool F4 = false>
 __device__ __forceinline__ float obc_far_I(float s2, float s3h, float u, float u2) {
   const float y = s2 * u;
-  float h = kObcH[kObcHDeg];
+  float h;
+  if (F4) {
+    h = kObcH4[kObcH4Deg];
 #pragma unroll
-  for (int k = kObcHDeg - 1; k >= 0; --k) h = fmaf(h, y, kObcH[k]);
+    for (int k = kObcH4Deg - 1; k >= 0; --k) h = fmaf(h, y, kObcH4[k]);
+  } else {
+    h = kObcH[kObcHDeg];
+#pragma unroll
+    for (int k = kObcHDeg - 1; k >= 0; --k) h = fmaf(h, y, kObcH[k]);
+  }
   return (s3h * u2) * h;
 }
 
 // -2 k(y): the factor -2 scales every coefficient exactly, so this is -2 times
 // the Horner value of k bit for bit.
+template <bool F4 = false>
 __device__ __forceinline__ float obc_far_km2(float y) {
-  float q = -2.0f * kObcK[kObcKDeg];
+  float q;
+  if (F4) {
+    q = -2.0f * kObcK4[kObcK4Deg];
 #pragma unroll
-  for (int k = kObcKDeg - 1; k >= 0; --k) q = fmaf(q, y, -2.0f * kObcK[k]);
+    for (int k = kObcK4Deg - 1; k >= 0; --k) q = fmaf(q, y, -2.0f * kObcK4[k]);
+  } else {
+    q = -2.0f * kObcK[kObcKDeg];
+#pragma unroll
+    for (int k = kObcKDeg - 1; k >= 0; --k) q = fmaf(q, y, -2.0f * kObcK[k]);
+  }
   return q;
 }
 
@@ -248,8 +266,9 @@ __device__ __forceinline__ float obc_far_dI(float s2, float s3h, float u, float 
 // Chain-rule scalar of a pair with both directions in the far field:
 //   (c_i dI(rho_i, s_j)/dr + c_j dI(rho_j, s_i)/dr) / r
 //     = -2 u^3 (c_i s_j^3/2 k(s_j^2 u) + c_j s_i^3/2 k(s_i^2 u)),   u = 1/r^2.
+template <bool F4 = false>
 __device__ __forceinline__ float chain_far(float u, float sj2, float ci_s3hj, float si2, float cj_s3hi) {
-  return (u * u * u) * fmaf(ci_s3hj, obc_far_km2(sj2 * u), cj_s3hi * obc_far_km2(si2 * u));
+  return (u * u * u) * fmaf(ci_s3hj, obc_far_km2<F4>(sj2 * u), cj_s3hi * obc_far_km2<F4>(si2 * u));
 }
 
 // ---------------------------------------------------------------------------
@@ -265,8 +284,9 @@ struct BornLds {  // (x, y, z, s^2), (s^3/2, rho, s, 0); j accumulators
 #endif
 };
 
-// FAR: every pair of the item is in the far field (no per-pair test).
-template <bool GENERAL, bool DIAG, bool FAR>
+// FAR: every pair of the item is in the far field (no per-pair test); F4:
+// and at least 4 s apart (the degree-4 fits).
+template <bool GENERAL, bool DIAG, bool FAR, bool F4 = false>
 __device__ __forceinline__ void born_tile(const DeviceView& v, const PairItem& it, const JSrc& js, int lane, int w,
                                           float cut2, BornLds& L) {
   const int ni = item_ni(it.meta), nj = js.nj;
@@ -298,8 +318,8 @@ __device__ __forceinline__ void born_tile(const DeviceView& v, const PairItem& i
     BornPair bp;
     if (FAR) {
       const float u = frcp(r2), u2 = u * u;
-      bp.I1 = obc_far_I(sj2, sj3h, u, u2);
-      bp.I2 = obc_far_I(bi.z, bi.w, u, u2);
+      bp.I1 = obc_far_I<F4>(sj2, sj3h, u, u2);
+      bp.I2 = obc_far_I<F4>(bi.z, bi.w, u, u2);
     } else {
       const float rinv = frsq(r2);
       const float u = rinv * rinv, u2 = u * u;
@@ -348,7 +368,8 @@ __device__ __forceinline__ void born_tile(const DeviceView& v, const PairItem& i
 // Class bits of an item from its tiles' bounding boxes (wave-uniform). The
 // OBC2 far-field bit needs the box distance above every pair's threshold,
 // max(s_j + rho_i, s_i + rho_j, 2 s) over the two tiles, with a 1e-4
-// relative margin on the squared distance.
+// relative margin on the squared distance; the s <= r/4 bit also needs it
+// above 4 s.
 __device__ __forceinline__ int classify(const DeviceView& v, const SlotConst& S, const PairItem& it) {
   const int tI = S.tile0 + ((it.i0 - S.a0) >> 6);
   const int tJ = S.tile0 + ((it.j0 - S.a0) >> 6);
@@ -364,6 +385,8 @@ __device__ __forceinline__ int classify(const DeviceView& v, const SlotConst& S,
   const float dmax2 = ex * ex + ey * ey + ez * ez;
   const float thr = fmaxf(fmaxf(hj.w + li.w, hi.w + lj.w), fmaxf(hi.w, hj.w) * (1.0f / PRX_OBC_FAR_XMAX));
   int cls = (tI != tJ && dmin2 > thr * thr * 1.0001f) ? kClsObcFar : 0;
+  const float thr4 = fmaxf(hi.w, hj.w) * (1.0f / PRX_OBC_FAR4_XMAX);
+  if (cls && dmin2 > thr4 * thr4 * 1.0001f) cls |= kClsObcFar4;
   if (S.cutoff2 > 0.0f) {
     if (dmin2 >= S.cutoff2 * 1.00001f)
       cls |= kClsFar;
@@ -433,6 +456,11 @@ __device__ __forceinline__ void born_item(const DeviceView& v, int w, BornLds& L
   const float cut2 = (S.gb_cut && ccut == kClsStraddle) ? S.cutoff2 : 0.0f;
   if (fl & kFlagDiag) {
     born_tile<true, true, false>(v, it, js, lane, w, cut2, L);
+  } else if (cls & kClsObcFar4) {
+    if (full && cut2 == 0.0f)
+      born_tile<false, false, true, true>(v, it, js, lane, w, 0.0f, L);
+    else
+      born_tile<true, false, true, true>(v, it, js, lane, w, cut2, L);
   } else if (cls & kClsObcFar) {
     if (full && cut2 == 0.0f)
       born_tile<false, false, true>(v, it, js, lane, w, 0.0f, L);
@@ -644,7 +672,7 @@ __device__ __forceinline__ void force_item(const DeviceView& v, int w, PairLds& 
 // Pass 3: chain rule through the Born radii
 // ---------------------------------------------------------------------------
 
-template <bool GENERAL, bool DIAG, bool FAR>
+template <bool GENERAL, bool DIAG, bool FAR, bool F4 = false>
 __device__ __forceinline__ void chain_tile(const DeviceView& v, const PairItem& it, const JSrc& js, int lane, int w,
                                            float cut2, PairLds& L) {
   const int ni = item_ni(it.meta), nj = js.nj;
@@ -676,7 +704,7 @@ __device__ __forceinline__ void chain_tile(const DeviceView& v, const PairItem& 
     const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
     float gsc;  // (c_i dI(r; rho_i, s_j)/dr + c_j dI(r; rho_j, s_i)/dr) / r; F_i = gsc (x_j - x_i)
     if (FAR) {
-      gsc = chain_far(frcp(r2), sj2, ci * sj3h, bi.z, cj * bi.w);
+      gsc = chain_far<F4>(frcp(r2), sj2, ci * sj3h, bi.z, cj * bi.w);
     } else {
       const float rinv = frsq(r2);
       const float rhoj = L.b[lane + k].z, sj = L.b[lane + k].w;  // near field only
@@ -744,6 +772,11 @@ __device__ __forceinline__ void chain_item(const DeviceView& v, int w, PairLds& 
   const bool full = item_ni(it.meta) == kTile && js.nj == kTile;
   if (fl & kFlagDiag) {
     chain_tile<true, true, false>(v, it, js, lane, w, cut2, L);
+  } else if (cls & kClsObcFar4) {
+    if (full && cut2 == 0.0f)
+      chain_tile<false, false, true, true>(v, it, js, lane, w, 0.0f, L);
+    else
+      chain_tile<true, false, true, true>(v, it, js, lane, w, cut2, L);
   } else if (cls & kClsObcFar) {
     if (full && cut2 == 0.0f)
       chain_tile<false, false, true>(v, it, js, lane, w, 0.0f, L);

@@ -89,13 +89,16 @@ def test_pair_integral_limits_and_far_field_identity():
                                                                      rel=1e-9, abs=1e-15)
 
 
-def test_far_field_polynomials_match_series():
-    """The coefficients compiled into the pair kernels reproduce h and k."""
+@pytest.mark.parametrize("suffix", ["", "4"])
+def test_far_field_polynomials_match_series(suffix):
+    """The coefficients compiled into the pair kernels reproduce h and k: the
+    fits on s <= r/2 and the degree-4 fits on s <= r/4."""
     txt = (ROOT / "prolix_amd/csrc/prx_obc_farfield.h").read_text()
-    H = [float(v.rstrip("f")) for v in re.search(r"kObcH\[\d+\] = \{(.*?)\}", txt).group(1).split(",")]
-    K = [float(v.rstrip("f")) for v in re.search(r"kObcK\[\d+\] = \{(.*?)\}", txt).group(1).split(",")]
-    ymax = float(re.search(r"PRX_OBC_FAR_YMAX ([0-9.]+)f", txt).group(1))
-    xmax = float(re.search(r"PRX_OBC_FAR_XMAX ([0-9.]+)f", txt).group(1))
+    H = [float(v.rstrip("f")) for v in re.search(rf"kObcH{suffix}\[\d+\] = \{{(.*?)\}}", txt).group(1).split(",")]
+    K = [float(v.rstrip("f")) for v in re.search(rf"kObcK{suffix}\[\d+\] = \{{(.*?)\}}", txt).group(1).split(",")]
+    ymax = float(re.search(rf"PRX_OBC_FAR{suffix}_YMAX ([0-9.]+)f", txt).group(1))
+    xmax = float(re.search(rf"PRX_OBC_FAR{suffix}_XMAX ([0-9.]+)f", txt).group(1))
+    assert (len(H), len(K)) == ((7, 8) if suffix == "" else (5, 5))
     assert xmax * xmax == pytest.approx(ymax, rel=1e-7)
     y = np.linspace(0.0, ymax, 2001)
     h = sum(2 * k / (2 * k + 1) * y ** (k - 1) for k in range(1, 300))
