@@ -12,21 +12,17 @@ namespace prx {
 // holds the value that started in lane (l + k) & 63.
 #define PRX_DPP_WAVE_ROL1 0x134
 
-// PRX_ROT_BPERM = 1 moves the value through the LDS crossbar
-// (ds_bpermute_b32: lane l reads lane (l + 1) & 63), which takes no VALU issue
-// slot; 0 uses the DPP wave rotate.
-#ifndef PRX_ROT_BPERM
-#define PRX_ROT_BPERM 0
-#endif
-
+// BPERM moves the value through the LDS crossbar (ds_bpermute_b32: lane l
+// reads lane (l + 1) & 63), which takes no VALU issue slot; otherwise the DPP
+// wave rotate. Chosen per pass (PRX_ROT_BPERM_BORN / _FORCE / _CHAIN below).
+template <bool BPERM>
 __device__ __forceinline__ float rot(float x) {
-#if PRX_ROT_BPERM
-  const int src = ((int)(threadIdx.x + 1) & 63) << 2;
-  return __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(x)));
-#else
+  if (BPERM) {
+    const int src = ((int)(threadIdx.x + 1) & 63) << 2;
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(x)));
+  }
   // Every lane has a valid source under a full-wave rotate, so no "old" value is needed.
   return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), PRX_DPP_WAVE_ROL1, 0xF, 0xF, true));
-#endif
 }
 
 // Partial-buffer slot of (item w, side, lane); side 0 = i atoms, 1 = j atoms.
@@ -59,6 +55,17 @@ __device__ __forceinline__ void wave_lds_sync() {
 // Settings chosen by the autotuner (prolix_amd/tune.py) for this GPU; the
 // defaults below apply to anything it leaves unset.
 #include "prx_tuned.h"
+
+// How each pass rotates its j accumulators: 1 = LDS crossbar, 0 = DPP (rot).
+#ifndef PRX_ROT_BPERM_BORN
+#define PRX_ROT_BPERM_BORN 0
+#endif
+#ifndef PRX_ROT_BPERM_FORCE
+#define PRX_ROT_BPERM_FORCE 0
+#endif
+#ifndef PRX_ROT_BPERM_CHAIN
+#define PRX_ROT_BPERM_CHAIN 0
+#endif
 
 // Work items (waves) per workgroup of the pair kernels.
 #ifndef PRX_PAIR_WAVES
@@ -351,7 +358,7 @@ __device__ __forceinline__ void born_tile(const DeviceView& v, const PairItem& i
     }
 #else
     accJ += bp.I2;
-    if (k != kend) accJ = rot(accJ);
+    if (k != kend) accJ = rot<PRX_ROT_BPERM_BORN>(accJ);
 #endif
   }
   float* pb = reinterpret_cast<float*>(v.part);  // scalar partials
@@ -600,7 +607,8 @@ __device__ __forceinline__ void force_tile(const DeviceView& v, const PairItem& 
     fxj = fmaf(fr, dx, fxj); fyj = fmaf(fr, dy, fyj); fzj = fmaf(fr, dz, fzj);
     dbj = fmaf(g, gbi.x, dbj);
     if (k != kend) {
-      fxj = rot(fxj); fyj = rot(fyj); fzj = rot(fzj); dbj = rot(dbj);
+      fxj = rot<PRX_ROT_BPERM_FORCE>(fxj); fyj = rot<PRX_ROT_BPERM_FORCE>(fyj);
+      fzj = rot<PRX_ROT_BPERM_FORCE>(fzj); dbj = rot<PRX_ROT_BPERM_FORCE>(dbj);
     }
 #endif
   }
@@ -738,7 +746,7 @@ __device__ __forceinline__ void chain_tile(const DeviceView& v, const PairItem& 
 #else
     fxj = fmaf(-gsc, dx, fxj); fyj = fmaf(-gsc, dy, fyj); fzj = fmaf(-gsc, dz, fzj);
     if (k != kend) {
-      fxj = rot(fxj); fyj = rot(fyj); fzj = rot(fzj);
+      fxj = rot<PRX_ROT_BPERM_CHAIN>(fxj); fyj = rot<PRX_ROT_BPERM_CHAIN>(fyj); fzj = rot<PRX_ROT_BPERM_CHAIN>(fzj);
     }
 #endif
   }

@@ -36,7 +36,7 @@ def test_plan_is_coordinate_descent_around_the_base():
 
 def test_header_round_trip():
     s = {"PRX_PAIR_WAVES": 8, "PRX_UNROLL_BORN": 4, "PRX_UNROLL_FORCE": 2, "PRX_UNROLL_CHAIN": 4, "PRX_XCD_MAP": 0,
-         "PRX_ROT_BPERM": 1, "PRX_MIN_WAVES": 6}
+         "PRX_ROT_BPERM_BORN": 0, "PRX_ROT_BPERM_FORCE": 1, "PRX_ROT_BPERM_CHAIN": 1, "PRX_MIN_WAVES": 6}
     assert tune.parse_tuned(tune.header(s, "a: 1.0\nb: 2.0")) == s
 
 
