@@ -237,8 +237,9 @@ int prx_engine_set_nl_skin(prx_engine* eng, double skin);
  * 3 = fused: one kernel per 1,000 steps in which one workgroup runs all the
  *     passes of one system with workgroup barriers only, systems side by
  *     side; PRX_ERR_STATE if a system uses a neighbour list;
- * 0 = auto (default): graphs, the fastest of the three on MI355X for every
- *     size measured (138 to 2,500 atoms).
+ * 0 = auto (default): fused when every system has at most 192 atoms and
+ *     none uses a neighbour list (measured fastest there on MI355X), graphs
+ *     otherwise (fastest from 256 to 2,500 atoms).
  * PRX_ERR_STATE for 2 where cooperative launches are unavailable. No
  * counterpart in prolix (its loop is jax.lax.fori_loop,
  * physics/simulate.py:253-254). */

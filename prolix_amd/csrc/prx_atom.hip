@@ -5,19 +5,22 @@
 namespace prx {
 // One block per tile: the gathers and finishes of prx_atom_dev.h.
 __global__ __launch_bounds__(256) void k_born_gather(DeviceView v) {
+  __shared__ GatherLds G;
   if ((int)blockIdx.x >= v.n_tiles) return;
-  born_gather_tile(v, blockIdx.x);
+  born_gather_tile(v, blockIdx.x, G);
 }
 
 template <bool ENERGY>
 __global__ __launch_bounds__(256) void k_force_gather(DeviceView v) {
+  __shared__ GatherLds G;
   if ((int)blockIdx.x >= v.n_tiles) return;
-  force_gather_tile<ENERGY>(v, blockIdx.x);
+  force_gather_tile<ENERGY>(v, blockIdx.x, G);
 }
 
 __global__ __launch_bounds__(256) void k_chain_gather(DeviceView v, int mode) {
+  __shared__ GatherLds G;
   if ((int)blockIdx.x >= v.n_tiles) return;
-  chain_gather_tile(v, blockIdx.x, mode);
+  chain_gather_tile(v, blockIdx.x, mode, G);
 }
 
 __global__ void k_step_begin(DeviceView v) { step_begin_atom(v, blockIdx.x * blockDim.x + threadIdx.x); }

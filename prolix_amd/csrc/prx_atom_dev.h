@@ -80,7 +80,7 @@ __device__ __forceinline__ TileCtx tile_ctx(const DeviceView& v, int t) {
   const int4 tl = v.tiles[t];
   TileCtx c;
   c.lane = threadIdx.x & 63;
-  c.q = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  c.q = __builtin_amdgcn_readfirstlane((int)((threadIdx.x >> 6) & (kGW - 1)));  // wave within its group of kGW
   c.g = tl.x + c.lane;
   c.valid = c.lane < tl.y;
   c.slot = tl.z;
@@ -265,17 +265,25 @@ __device__ __forceinline__ double4 gather4(const DeviceView& v, const TileCtx& c
   return acc;
 }
 
+// Workgroup memory of one tile gather: the four waves' sums and the bonded
+// shares. The kernel that runs the gathers owns it; a workgroup that runs
+// several tile gathers side by side gives each group of four waves its own.
+struct GatherLds {
+  double4 sh4[kGW][kTile];
+  float4 shb[kGW][kTile];
+};
+
 // The four waves' sums, added in wave order; valid in wave 0 after the call.
 // Every thread of the block must call it (it holds a barrier).
-__device__ __forceinline__ double combine1(double x, const TileCtx& c) {
-  __shared__ double sh[kGW][kTile];
+__device__ __forceinline__ double combine1(double x, const TileCtx& c, GatherLds& G) {
+  double(*sh)[kTile] = reinterpret_cast<double(*)[kTile]>(&G.sh4[0][0]);
   sh[c.q][c.lane] = x;
   __syncthreads();
   return ((sh[0][c.lane] + sh[1][c.lane]) + sh[2][c.lane]) + sh[3][c.lane];
 }
 
-__device__ __forceinline__ double4 combine4(double4 x, const TileCtx& c) {
-  __shared__ double4 sh4[kGW][kTile];
+__device__ __forceinline__ double4 combine4(double4 x, const TileCtx& c, GatherLds& G) {
+  double4(*sh4)[kTile] = G.sh4;
   sh4[c.q][c.lane] = x;
   __syncthreads();
   double4 r = sh4[0][c.lane];
@@ -300,10 +308,10 @@ __device__ __forceinline__ bool tile_done(const DeviceView& v, int t) {
 
 // B_i from I_i in FP64 (generalized_born.py:81-93); dB/dI for the chain rule.
 // Neighbour-list GB systems floor 1/B at 1e-4 (generalized_born.py:333-337).
-__device__ __forceinline__ void born_gather_tile(const DeviceView& v, int t) {
+__device__ __forceinline__ void born_gather_tile(const DeviceView& v, int t, GatherLds& G) {
   if (tile_done(v, t)) return;
   const TileCtx c = tile_ctx(v, t);
-  const double I = combine1(gather1(v, c), c);
+  const double I = combine1(gather1(v, c), c, G);
   if (c.q != 0 || !c.valid) return;
   const double4 f = v.fin4[c.g];  // rho, R, q^2, ace K
   const double psi = f.x * I;
@@ -319,10 +327,10 @@ __device__ __forceinline__ void born_gather_tile(const DeviceView& v, int t) {
 
 // Pair force and dE/dB; c_i = dE/dB_i * dB_i/dI_i with the GB self term and ACE (FP64).
 template <bool ENERGY>
-__device__ __forceinline__ void force_gather_tile(const DeviceView& v, int t) {
+__device__ __forceinline__ void force_gather_tile(const DeviceView& v, int t, GatherLds& G) {
   if (tile_done(v, t)) return;
   const TileCtx c = tile_ctx(v, t);
-  const double4 a = combine4(gather4(v, c), c);
+  const double4 a = combine4(gather4(v, c), c, G);
   if (c.q != 0 || !c.valid) return;
   const double4 f = v.fin4[c.g];
   const float4 gb = v.gb4[c.g];
@@ -459,18 +467,17 @@ __device__ __forceinline__ float4 finish_atom(const DeviceView& v, const TileCtx
 // moved to the next step's positions, and wave 0 also writes the tile's box
 // for that step (what k_tile_boxes computes), so the step graph needs no box
 // kernel after its first step.
-__device__ __forceinline__ void chain_gather_tile(const DeviceView& v, int t, int mode) {
+__device__ __forceinline__ void chain_gather_tile(const DeviceView& v, int t, int mode, GatherLds& G) {
   if (tile_done(v, t)) return;
   const TileCtx c = tile_ctx(v, t);
   double4 part = gather3(v, c);
   float3 fb = c.valid ? bonded_force(v, c.g, c.q) : make_float3(0.f, 0.f, 0.f);
   // the bonded share travels in the same combine (its float sum is exact in FP64)
-  __shared__ float4 shb[kGW][kTile];
-  shb[c.q][c.lane] = make_float4(fb.x, fb.y, fb.z, 0.0f);
-  const double4 a = combine4(part, c);
+  G.shb[c.q][c.lane] = make_float4(fb.x, fb.y, fb.z, 0.0f);
+  const double4 a = combine4(part, c, G);
   if (c.q != 0) return;
   float4 x = make_float4(0.f, 0.f, 0.f, 0.f);
-  if (c.valid) x = finish_atom(v, c, a, shb, mode);
+  if (c.valid) x = finish_atom(v, c, a, G.shb, mode);
   if (mode != 1 || !v.slots[c.slot].active) return;  // uniform over the tile
   const int last = v.tiles[t].y - 1;
   const float xl = __shfl(x.x, last), yl = __shfl(x.y, last), zl = __shfl(x.z, last);

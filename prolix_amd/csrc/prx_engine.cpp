@@ -352,10 +352,14 @@ struct prx_engine {
   int32_t* d_nl_colitems = nullptr;
   int64_t cap_colitems = 0;
   bool has_nl = false;            // a live system uses its neighbour list
-  // step path: 0 = auto (graphs: on MI355X neither kernel below beats them,
-  // measured from 138 to 2,500 atoms), 1 = captured graphs, 2 = the
-  // persistent kernel, 3 = the fused kernel (one workgroup per system)
+  // step path: 0 = auto (the fused kernel when every system fits in
+  // kFusedAutoAtoms atoms and none uses a neighbour list, else graphs: on
+  // MI355X the fused kernel beats graphs only there, measured from 138 to
+  // 2,500 atoms), 1 = captured graphs, 2 = the persistent kernel, 3 = the
+  // fused kernel (one workgroup per system)
   int32_t step_mode = 0;
+  static constexpr int32_t kFusedAutoAtoms = 3 * prx::kTile;
+  int32_t max_system_atoms = 0;   // largest live system (rebuild)
   prx::GridBarrier* d_bar = nullptr;
   int persist_blocks = -1;        // co-resident grid of the persistent kernel (0: unavailable)
   // pool slots admitted since the last step call: their momenta are drawn and
@@ -877,6 +881,7 @@ struct prx_engine {
     std::vector<int2> nl_col;
     std::vector<int32_t> nl_colitems;
     offsets.assign(1, 0);
+    max_system_atoms = 0;
     has_nl = false;
     for (int32_t sid : order) {
       Slot& s = slots[sid];
@@ -909,6 +914,7 @@ struct prx_engine {
       ranges.insert(ranges.end(), r, r + 14);
       for (int i = 0; i < s.n; ++i) map.push_back(s.a0 + i);
       offsets.push_back(offsets.back() + s.n);
+      max_system_atoms = std::max(max_system_atoms, s.n);
     }
     if ((int64_t)items.size() > cap_items) {
       if (arena) throw std::logic_error("arena engine: work list larger than sized");
@@ -1104,6 +1110,7 @@ struct prx_engine {
   // The fused kernel: every system in one workgroup of its own. Not for
   // neighbour-list systems (their list build spans workgroups).
   bool use_fused() const {
+    if (step_mode == 0) return !has_nl && max_system_atoms <= kFusedAutoAtoms;
     if (step_mode != 3) return false;
     if (has_nl) throw StateError("the fused step kernel does not run neighbour-list systems");
     return true;

@@ -70,6 +70,7 @@ __global__ __launch_bounds__(kPersistThreads) void k_persist_steps(DeviceView v,
                                                                     int n_steps, int has_nl, GridBarrier* bar) {
   __shared__ PassLds lds[kPersistWaves];
   PassLds& L = lds[threadIdx.x >> 6];
+  __shared__ GatherLds G;
   unsigned gen = 0;  // the barrier's generation, tracked by thread 0
   const int wave = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * kPersistWaves + (threadIdx.x >> 6)));
   const int n_waves = gridDim.x * kPersistWaves;
@@ -101,14 +102,14 @@ __global__ __launch_bounds__(kPersistThreads) void k_persist_steps(DeviceView v,
     for (int e = tid; e < n_bonded; e += n_threads) bonded_entry<false>(v, e);
     if (!grid_sync(bar, gen)) return;
     for (int t = blockIdx.x; t < v.n_tiles; t += gridDim.x) {
-      born_gather_tile(v, t);
+      born_gather_tile(v, t, G);
       __syncthreads();
     }
     if (!grid_sync(bar, gen)) return;
     for (int w = wave; w < v.n_items; w += n_waves) force_item<false>(v, w, L.pair);
     if (!grid_sync(bar, gen)) return;
     for (int t = blockIdx.x; t < v.n_tiles; t += gridDim.x) {
-      force_gather_tile<false>(v, t);
+      force_gather_tile<false>(v, t, G);
       __syncthreads();
     }
     if (!grid_sync(bar, gen)) return;
@@ -116,7 +117,7 @@ __global__ __launch_bounds__(kPersistThreads) void k_persist_steps(DeviceView v,
     if (!grid_sync(bar, gen)) return;
     const int mode = k + 1 == n_steps ? 2 : 1;  // the last step stops after B(F)
     for (int t = blockIdx.x; t < v.n_tiles; t += gridDim.x) {
-      chain_gather_tile(v, t, mode);
+      chain_gather_tile(v, t, mode, G);
       __syncthreads();
     }
     if (k + 1 < n_steps && !grid_sync(bar, gen)) return;

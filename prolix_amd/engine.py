@@ -160,7 +160,8 @@ class GBEngine:
         per-pass kernels), "persistent" (one cooperative kernel per 1,000
         steps, grid-wide barriers), "fused" (one workgroup per system,
         workgroup barriers only; not for neighbour-list systems) or "auto"
-        (the default: graphs, the fastest of the three on MI355X)."""
+        (the default: fused when every system has at most 192 atoms and none
+        uses a neighbour list, graphs otherwise; the fastest on MI355X)."""
         N.check(N.lib().prx_engine_set_step_mode(self._h, self.STEP_MODES[mode]), "prx_engine_set_step_mode")
 
     def set_fork_min_atoms(self, min_atoms: int):

@@ -3,7 +3,8 @@ barriers only) against the captured-graph path: the same per-item and
 per-tile device code on the same work lists, so trajectories agree bit for
 bit -- one small system, several packed systems of different sizes, a
 1,000-atom system, and a pool whose systems stop at their step limits. The
-wall time per step of both paths is printed; the auto mode runs graphs."""
+wall time per step of both paths is printed; the auto mode runs the fused
+kernel up to 192 atoms per system and graphs above."""
 
 from __future__ import annotations
 
