@@ -175,6 +175,8 @@ def main():
     ap.add_argument("--n-systems", type=int, default=256)
     ap.add_argument("--min-steps", type=int, default=2000,
                     help="steepest-descent steps before dynamics (mix and the cutoff system)")
+    ap.add_argument("--lbfgs-steps", type=int, default=2000,
+                    help="L-BFGS steps after the steepest descent (the batched minimiser's last stage)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=32)
@@ -226,14 +228,16 @@ def main():
     import torch
 
     from prolix_amd.engine import GBEngine, Langevin
-    from prolix_amd.minimize import minimize
+    from prolix_amd.minimize import lbfgs, minimize
 
     # 2. engine, relaxation, thermostat start
     eng = GBEngine(systems, cutoff=0.0)
     t0 = time.time()
     x = minimize(eng, [s["positions"] for s in systems], steps=a.min_steps)
+    if a.lbfgs_steps > 0:
+        x = lbfgs(eng, x, steps=a.lbfgs_steps)
     torch.cuda.synchronize()
-    log(f"minimised {a.min_steps} steps in {time.time() - t0:.1f}s")
+    log(f"minimised {a.min_steps} steepest-descent + {a.lbfgs_steps} L-BFGS steps in {time.time() - t0:.1f}s")
     params = [Langevin(DT_FS, float(temps[k]), 1.0, seed=1000 + k, safety_caps=True) for k in range(len(systems))]
     eng.init(x, params)
     eng.steps(a.warmup)
@@ -327,6 +331,8 @@ def main():
         rc = 20.0
         e2 = GBEngine(ls, cutoff=rc, gb_cutoff=True)
         xl = minimize(e2, ls["positions"], steps=a.min_steps)
+        if a.lbfgs_steps > 0:
+            xl = lbfgs(e2, xl, steps=a.lbfgs_steps)
         e2.init(xl, Langevin(DT_FS, 300.0, 1.0, seed=77, safety_caps=True))
         e2.steps(20)
         torch.cuda.synchronize()
@@ -491,7 +497,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "batched mix: 256 synthetic proteins, N log-uniform [300, 5000], no cutoff, "
                                "BAOAB dt 1 fs, gamma 1/ps, T U(280,350) K",
-                   "n_systems": len(systems), "atoms": int(sizes.sum()), "pairs_per_step": pairs_per_step},
+                   "n_systems": len(systems), "atoms": int(sizes.sum()), "pairs_per_step": pairs_per_step,
+                   "relaxation": f"{a.min_steps} steepest-descent + {a.lbfgs_steps} L-BFGS steps before dynamics"},
         "pair_interactions_per_s": pairs_per_step * sps,
         "latency_p50_s": serving["latency_p50_s"] if serving else None,
         "latency_p99_s": serving["latency_p99_s"] if serving else None,
