@@ -15,6 +15,15 @@ import numpy as np
 FRACTIONS = (1.0, 0.5, 0.25, 0.1)
 
 
+def _mass_tensor(engine, masses):
+    """Packed (N,) masses in amu as a float32 device tensor, from an array,
+    a list or a tensor on any device."""
+    t = engine.torch
+    if isinstance(masses, t.Tensor):
+        return masses.detach().reshape(-1).to(device=engine.device, dtype=t.float32)
+    return t.as_tensor(np.asarray(masses, np.float64).reshape(-1), dtype=t.float32, device=engine.device)
+
+
 def _lengths(engine):
     """Per-system atom counts as a device tensor: the packed layout keeps each
     system's atoms contiguous, so per-system reductions are segment reductions
@@ -189,7 +198,7 @@ def fire_step(engine, state, masses, dt_max, force_cap=None, disp_cap=0.1, force
 def fire(engine, positions, steps, masses, dt_start, dt_max, force_cap=None, disp_cap=0.1, state=None, **kw):
     """Run ``steps`` FIRE steps; returns the final FireState."""
     t = engine.torch
-    mt = t.as_tensor(np.asarray(masses, np.float64).reshape(-1), dtype=t.float32, device=engine.device)
+    mt = _mass_tensor(engine, masses)
     st = state if state is not None else fire_init(engine, positions, dt_start, kw.get("alpha_start", 0.1))
     for _ in range(int(steps)):
         st = fire_step(engine, st, mt, dt_max, force_cap, disp_cap, **kw)
@@ -267,7 +276,7 @@ def restrained_fire(engine, positions, masses, steps, k_restraint=0.0, selection
             f = f - float(k_restraint) * (x - ref) * sel[:, None]
         return t.where(t.isfinite(f), f, t.zeros_like(f))
 
-    mt = t.as_tensor(np.asarray(masses, np.float64).reshape(-1), dtype=t.float32, device=engine.device)
+    mt = _mass_tensor(engine, masses)
     ns = engine.n_systems
     st = FireState(x0, t.zeros_like(x0), force_fn(x0), t.full((ns,), float(dt_start), dtype=t.float64, device=x0.device),
                    t.full((ns,), 0.1, dtype=t.float64, device=x0.device), t.zeros((ns,), dtype=t.int64, device=x0.device))

@@ -153,3 +153,25 @@ def test_batched_minimize_stages_per_system():
         one.close()
         off += n
         assert np.max(np.abs(xa.cpu().numpy() - xb[k])) < 1e-4
+
+
+def test_masses_as_device_tensor():
+    """The FIRE stages take the masses as an array or as a tensor on the
+    device, with the same result."""
+    import torch
+
+    from prolix_amd.engine import GBEngine
+    from prolix_amd.minimize import fire, restrained_fire
+
+    s = protein(60, 4)
+    m = np.asarray(s["masses"], np.float64)
+    mt = torch.as_tensor(m, device="cuda")
+    print("[gpu] FIRE with masses as array and as device tensor", flush=True)
+    eng = GBEngine(s)
+    a = fire(eng, s["positions"], 10, m, 1e-4, 1e-3).positions.cpu().numpy()
+    b = fire(eng, s["positions"], 10, mt, 1e-4, 1e-3).positions.cpu().numpy()
+    ra, _ = restrained_fire(eng, s["positions"], m, 10, k_restraint=25.0)
+    rb, _ = restrained_fire(eng, s["positions"], mt, 10, k_restraint=25.0)
+    eng.close()
+    assert np.array_equal(a, b)
+    assert np.array_equal(ra.cpu().numpy(), rb.cpu().numpy())
