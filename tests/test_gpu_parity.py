@@ -119,6 +119,8 @@ def test_two_atoms_finite_difference_geometry():
 def test_small_proteins(n_atoms, seed):
     sysd = protein(n_atoms, seed)
     _, _, _, _, _, info = _compare(f"protein_{n_atoms}", sysd)
+    if n_atoms == 138:  # the 138-atom peptide: the error is relative to |E_total| itself
+        assert abs(info["total_oracle"]) >= 0.2 * info["term_scale"]
     assert info["total_rel"] < E_TOL
     assert info["force_rel_rms"] < F_TOL
     assert info["born_rel_max"] < 1e-5
@@ -129,6 +131,7 @@ def test_small_proteins(n_atoms, seed):
 def test_globule_2500():
     sysd = protein(2500, 1, radius=synth.compact_radius(2500))
     _, _, _, _, _, info = _compare("globule_2500", sysd)
+    assert abs(info["total_oracle"]) >= 0.2 * info["term_scale"]  # relative to |E_total| itself
     assert info["total_rel"] < E_TOL
     assert info["force_rel_rms"] < F_TOL
     assert info["born_rel_max"] < 1e-5
