@@ -210,6 +210,7 @@ void launch_nl_build(const DeviceView& v, const int32_t* slot_ranges, int n_slot
 void launch_dense_list(const float* pos, int n, float radius, int capacity, int32_t* idx, int32_t* counts,
                        float4* boxes, hipStream_t s);
 void launch_born_pairs(const DeviceView& v, hipStream_t s);                // also -> item_cls
+void launch_born_pairs_bonded(const DeviceView& v, hipStream_t s);         // the same, and launch_bonded (forces only)
 void launch_born_gather(const DeviceView& v, hipStream_t s);               // -> B, dB/dI
 void launch_force_pairs(const DeviceView& v, bool energy, hipStream_t s);
 void launch_force_gather(const DeviceView& v, bool energy, hipStream_t s);  // -> chain factors, pair force

@@ -1027,7 +1027,13 @@ struct prx_engine {
     // every list is rebuilt where the boxes are (eager evaluations, the first
     // step of a graph launch); later steps rebuild by the skin rule
     if (has_nl) prx::launch_nl_build(vv, d_ranges, (int)order.size(), boxes, s);
-    prx::launch_born_pairs(vv, s);
+    // in line, the bonded forces ride in the Born pass's launch (one graph
+    // node less per step); energy evaluations run the bonded kernel
+    const bool merged = !fork && !energy;
+    if (merged)
+      prx::launch_born_pairs_bonded(vv, s);
+    else
+      prx::launch_born_pairs(vv, s);
     prx::launch_born_gather(vv, s);
     prx::launch_force_pairs(vv, energy, s);
     prx::launch_force_gather(vv, energy, s);
@@ -1035,7 +1041,7 @@ struct prx_engine {
       prx::launch_chain_pairs(vv, s);
       HIPCHECK(hipStreamWaitEvent(s, ev_join, 0));
     } else {
-      prx::launch_bonded(vv, energy, s);
+      if (!merged) prx::launch_bonded(vv, energy, s);
       prx::launch_chain_pairs(vv, s);
     }
   }

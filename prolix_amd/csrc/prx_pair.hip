@@ -29,6 +29,7 @@
 // and the gathers). Far items skip LJ and Coulomb, and with the neighbour-list
 // GB form (generalized_born.py:266-415, gb_cut) skip every pass; straddling
 // items test r^2 < cutoff^2 per pair.
+#include "prx_bonded_dev.h"
 #include "prx_pair_dev.h"
 
 namespace prx {
@@ -54,7 +55,41 @@ __global__ __launch_bounds__(kPairThreads) PRX_OCCUPANCY void k_chain_pairs(Devi
   chain_item(v, w, lds[threadIdx.x >> 6]);
 }
 
+// The Born pass with the bonded terms in the same launch: blocks below
+// n_pair are the pair kernel's (its XCD-aware item order over those blocks),
+// the rest take one bonded entry per thread. Both read positions only and the
+// bonded records are read by the chain gather, so the merged launch saves the
+// bonded kernel's graph node where that kernel would run in line; results are
+// bit for bit those of the two kernels.
+__device__ __forceinline__ int wave_id_of(int b, int nb) {
+#if PRX_XCD_MAP
+  const int per = nb / kXcds, rem = nb % kXcds;
+  const int x = b % kXcds, k = b / kXcds;
+  b = x * per + min(x, rem) + k;
+#endif
+  return __builtin_amdgcn_readfirstlane((int)(b * kWavesPerBlock + (threadIdx.x >> 6)));
+}
+
+__global__ __launch_bounds__(kPairThreads) void k_born_pairs_bonded(DeviceView v, int n_pair) {
+  __shared__ BornLds lds[kWavesPerBlock];
+  const int b = (int)blockIdx.x;
+  if (b < n_pair) {
+    const int w = wave_id_of(b, n_pair);
+    if (w >= v.n_items) return;
+    born_item(v, w, lds[threadIdx.x >> 6]);
+    return;
+  }
+  const int e = (b - n_pair) * kPairThreads + (int)threadIdx.x;
+  if (e < bonded_entries(v)) bonded_entry<false>(v, e);
+}
+
 static inline int pair_blocks(int n_items) { return (n_items + kWavesPerBlock - 1) / kWavesPerBlock; }
+
+void launch_born_pairs_bonded(const DeviceView& v, hipStream_t s) {
+  const int np = v.n_items ? pair_blocks(v.n_items) : 0;
+  const int nb = (bonded_entries(v) + kPairThreads - 1) / kPairThreads;
+  if (np + nb > 0) hipLaunchKernelGGL(k_born_pairs_bonded, dim3(np + nb), dim3(kPairThreads), 0, s, v, np);
+}
 
 void launch_born_pairs(const DeviceView& v, hipStream_t s) {
   if (v.n_items == 0) return;
