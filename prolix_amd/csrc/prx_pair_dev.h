@@ -56,7 +56,8 @@ __device__ __forceinline__ void wave_lds_sync() {
 // defaults below apply to anything it leaves unset.
 #include "prx_tuned.h"
 
-// How each pass rotates its j accumulators: 1 = LDS crossbar, 0 = DPP (rot).
+// How each pass rotates its j accumulators: 1 = LDS crossbar, 0 = DPP (rot);
+// the chain pass's three also split 1 + 2 (value 2) or 2 + 1 (value 3).
 #ifndef PRX_ROT_BPERM_BORN
 #define PRX_ROT_BPERM_BORN 0
 #endif
@@ -746,7 +747,10 @@ __device__ __forceinline__ void chain_tile(const DeviceView& v, const PairItem& 
 #else
     fxj = fmaf(-gsc, dx, fxj); fyj = fmaf(-gsc, dy, fyj); fzj = fmaf(-gsc, dz, fzj);
     if (k != kend) {
-      fxj = rot<PRX_ROT_BPERM_CHAIN>(fxj); fyj = rot<PRX_ROT_BPERM_CHAIN>(fyj); fzj = rot<PRX_ROT_BPERM_CHAIN>(fzj);
+      // PRX_ROT_BPERM_CHAIN 2 or 3: the first one or two by the crossbar, the rest by DPP
+      fxj = rot<(PRX_ROT_BPERM_CHAIN >= 1)>(fxj);
+      fyj = rot<(PRX_ROT_BPERM_CHAIN == 1 || PRX_ROT_BPERM_CHAIN == 3)>(fyj);
+      fzj = rot<(PRX_ROT_BPERM_CHAIN == 1)>(fzj);
     }
 #endif
   }
